@@ -1,0 +1,260 @@
+#!/usr/bin/env python3
+"""Benchmark of the 3-D variable-coefficient heat update on one MI355X.
+
+Workload (the FP64 GPU configuration the headline metric is quoted on):
+512^3 interior + 1-cell Dirichlet ring (514^3 points), seeded sine kappa
+(1 + 0.5 sin sin sin) x U[0.9,1.1], Gaussian hot spot u0 + U[0,1e-3],
+dt = 0.9 x the FP64 positivity bound, FP64 storage and arithmetic.  One
+"step" is one time step of the whole field; --steps of them are timed
+(default 1000, as in the configuration), after --warmup untimed steps.
+
+Reported beside it, in the same run:
+  - the measured HBM stream bandwidth (copy / triad probe on 4 GiB arrays),
+  - the FP32-compensated, temporally blocked kernel on the same field and
+    step count, with its GLUP/s, % of stream bandwidth, speed-up over the
+    FP64 kernel and rel-L2 / max error against an FP64 run on the same
+    FP32-rounded inputs,
+  - the CPU baseline: the reference's own serial program (built from its
+    source into oracle/_ref) with OpenMP on this host's cores, on the
+    reference's recorded CPU case (n = 256, 101 steps), and the plain C
+    oracle on one core on 128^3 x 100 steps.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(REPO, "openmp-offloading-intranode-artifact_amd")
+sys.path.insert(0, PKG)
+sys.path.insert(0, os.path.join(REPO, "tests"))
+
+import numpy as np  # noqa: E402
+
+import h3d_py as h3d  # noqa: E402
+
+METRIC = "GLUP/s + % HBM roofline; FP32-compensated vs FP64 at stated rel-L2, 1 MI355X"
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--m", type=int, default=512, help="interior points per axis")
+    ap.add_argument("--kblock", type=int, default=0,
+                    help="FP32 time steps per pass (0: autotune)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    return ap.parse_args()
+
+
+def traffic_from_profiles(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE,
+    WRITE_SIZE, scaled by the copy-kernel calibration), committed under
+    profiles/; None when not measured for this kernel and shape."""
+    path = os.path.join(REPO, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        return json.load(open(path)).get(name, {}).get("bytes_per_launch")
+    except (OSError, ValueError):
+        return None
+
+
+def cpu_baseline():
+    """The reference's serial program (OpenMP) on the reference's recorded
+    CPU case, and the oracle on one core; both bounded to a few seconds."""
+    out = {}
+    ncores = len(os.sched_getaffinity(0))
+    threads = min(ncores, 16)
+    ref = os.path.join(REPO, "oracle", "_ref", "serial")
+    n, s = 256, 100
+    h = 1.0 / (n - 1)
+    dt = h * h / (12.0 * 0.95)
+    T = "%.7g" % (s * dt)  # submit.sh:61-62 -> 101 steps
+    if os.path.exists(ref):
+        env = dict(os.environ, WRITE_TXT="0", OMP_NUM_THREADS=str(threads),
+                   OMP_PROC_BIND="spread", OMP_PLACES="cores")
+        r = subprocess.run([ref, str(n), T], env=env, capture_output=True, text=True,
+                           timeout=300, cwd="/tmp")
+        steps = int(re.search(r"^(\d+) timesteps complete", r.stdout, re.M).group(1))
+        sec = float(re.search(r"^Time = ([0-9.]+) seconds", r.stdout, re.M).group(1))
+        glups = (n - 2) ** 3 * steps / sec / 1e9
+        out = {"value": glups, "unit": "GLUP/s", "cores": threads, "kind": "reference",
+               "sample": f"reference heat3D_variable_coeff.c built from source, "
+                         f"./serial {n} {T} ({steps} steps, {n - 2}^3 interior), "
+                         f"OMP_NUM_THREADS={threads}, {sec:.3f} s",
+               "host_cpus": ncores}
+    # one-core C oracle on the 128^3 configuration (port of the same loop)
+    import oracle_py as oracle
+    m = 128
+    nn = m + 2
+    g = h3d.Grid.dense(nn)
+    u, k = h3d.gen_seeded(g, 0, nn, nn, nn, 0, 20260101)
+    fac = 0.9 * 2.0 / oracle.face_sum_max(k, nn, nn, nn) / 2.0
+    t0 = time.perf_counter()
+    oracle.run_f64(u.copy(), u.copy(), None, k, nn, nn, nn, 100, factor=fac, nthreads=1)
+    sec = time.perf_counter() - t0
+    serial = {"value": m ** 3 * 100 / sec / 1e9, "unit": "GLUP/s", "cores": 1,
+              "kind": "port", "sample": f"C oracle, 128^3 interior x 100 steps, {sec:.3f} s"}
+    if not out:
+        out = dict(serial)
+    out["serial"] = serial
+    return out
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); "
+              "the z-slab path would shard across GPUs as described in DESIGN.md, "
+              "but no multi-GPU figure is produced.", file=sys.stderr)
+        sys.exit(2)
+    if h3d.device_count() < 1:
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    arch = (h3d.C.c_char * 64)()
+    h3d.call("h3d_device_arch", 0, arch, 64)
+    log("device", arch.value.decode())
+
+    # ---- HBM stream bandwidth (the roofline) ----
+    copy_gbs, triad_gbs = h3d.stream_probe(4 << 30, reps=5)
+    bw = max(copy_gbs, triad_gbs)
+    log(f"stream: copy {copy_gbs:.0f} GB/s, triad {triad_gbs:.0f} GB/s")
+
+    # ---- problem ----
+    m = a.m
+    n = m + 2
+    g = h3d.Grid.dense(n)
+    t0 = time.perf_counter()
+    u0, kap = h3d.gen_seeded(g, 0, n, n, n, 0, 20260101)
+    hh = 1.0 / (n - 1)
+    fmax = h3d.lib().h3d_face_sum_max_host(h3d.ptr(kap), h3d.C.byref(g))
+    dt = 0.9 * 2.0 * hh * hh / fmax
+    factor = dt / hh / hh / 2.0
+    log(f"generated {n}^3 fields in {time.perf_counter() - t0:.1f} s; dt={dt:.6e}")
+
+    stream = h3d.Stream()
+    da, db, dk = (h3d.DeviceArray.from_host(x) for x in (u0, u0, kap))
+    tune = h3d.Tune()
+    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0,
+             stream.ptr, h3d.C.byref(tune))
+    log(f"FP64 tune: rows={tune.rows} waves={tune.waves} zchunk={tune.zchunk}")
+
+    ms, fb = h3d.C.c_float(), h3d.C.c_int()
+    if a.warmup:
+        h3d.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor,
+                 a.warmup, h3d.C.byref(tune), 0, stream.ptr, h3d.C.byref(ms),
+                 h3d.C.byref(fb))
+    # fresh initial state for the timed run
+    da.upload(u0)
+    db.upload(u0)
+    h3d.call("h3d_device_synchronize", 0)
+    t0 = time.perf_counter()
+    h3d.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor,
+             a.steps, h3d.C.byref(tune), 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+    h3d.call("h3d_device_synchronize", 0)
+    wall = time.perf_counter() - t0
+    dev_s = ms.value / 1e3
+    cells = m ** 3
+    glups = cells * a.steps / wall / 1e9
+    per_launch_s = dev_s / a.steps
+    alg_bytes = 24.0 * cells  # u_old + kappa read, u_new written
+    achieved = alg_bytes / per_launch_s / 1e9
+    res64 = db if fb.value else da
+    st = h3d.stats(res64, g)
+    log(f"FP64: {a.steps} steps in {wall:.3f} s (events {dev_s:.3f} s) -> {glups:.1f} GLUP/s, "
+        f"{achieved:.0f} GB/s ({100 * achieved / bw:.1f}% of stream)")
+
+    result = {
+        "metric": METRIC, "value": round(glups, 3), "unit": "GLUP/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(1e3 * wall / a.steps, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded generators)",
+        "config": {"workload": f"heat3d_variable_coeff FP64, {m}^3 interior + Dirichlet ring, "
+                               f"seeded sine kappa, Gaussian u0, dt=0.9*bound, rhs=0",
+                   "interior": [m, m, m], "points": [n, n, n], "seed": 20260101,
+                   "tune": {"rows": tune.rows, "waves": tune.waves, "zchunk": tune.zchunk}},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(f"h3d_march_f64:{m}"),
+                     "frac_of_stream": round(achieved / bw, 4),
+                     "algorithmic_bytes_per_lup": 24,
+                     "kernel_ms_per_launch": round(1e3 * per_launch_s, 5)},
+        "stream": {"copy_gbs": round(copy_gbs, 1), "triad_gbs": round(triad_gbs, 1),
+                   "array_bytes": 4 << 30},
+        "checksum": {"sum": st[0], "sumsq": st[1], "max": st[2]},
+    }
+
+    # ---- FP32 compensated, temporally blocked ----
+    if not a.no_fp32:
+        u32 = u0.astype(np.float32)
+        k32 = kap.astype(np.float32)
+        # FP64 reference on the FP32-rounded inputs (untimed)
+        da.upload(u32.astype(np.float64))
+        db.upload(u32.astype(np.float64))
+        dk.upload(k32.astype(np.float64))
+        h3d.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor,
+                 a.steps, h3d.C.byref(tune), 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+        ref = db if fb.value else da
+        fa, fbuf, fk = (h3d.DeviceArray.from_host(x) for x in (u32, u32, k32))
+        f32f = h3d.C.c_float(np.float32(factor))
+        t32 = h3d.Tune()
+        h3d.call("h3d_autotune", 1, fa.ptr, fbuf.ptr, fk.ptr, 0, h3d.C.byref(g), 1,
+                 a.kblock, stream.ptr, h3d.C.byref(t32))
+        log(f"FP32 tune: kblock={t32.kblock} rows={t32.rows} waves={t32.waves} "
+            f"zchunk={t32.zchunk}")
+        h3d.call("h3d_run_f32", fa.ptr, fbuf.ptr, fk.ptr, None, None, None, h3d.C.byref(g),
+                 f32f, max(a.warmup, t32.kblock), 1, h3d.C.byref(t32), 0, stream.ptr,
+                 h3d.C.byref(ms), h3d.C.byref(fb))
+        fa.upload(u32)
+        fbuf.upload(u32)
+        h3d.call("h3d_device_synchronize", 0)
+        t0 = time.perf_counter()
+        h3d.call("h3d_run_f32", fa.ptr, fbuf.ptr, fk.ptr, None, None, None, h3d.C.byref(g),
+                 f32f, a.steps, 1, h3d.C.byref(t32), 0, stream.ptr, h3d.C.byref(ms),
+                 h3d.C.byref(fb))
+        h3d.call("h3d_device_synchronize", 0)
+        wall32 = time.perf_counter() - t0
+        glups32 = cells * a.steps / wall32 / 1e9
+        passes = -(-a.steps // t32.kblock)
+        bytes32 = 12.0 * cells * passes  # u + kappa read, u written, per pass
+        gbs32 = bytes32 / (ms.value / 1e3) / 1e9
+        m32 = h3d.compare(ref, fbuf if fb.value else fa, g, f32=True)
+        log(f"FP32c: {glups32:.1f} GLUP/s, {gbs32:.0f} GB/s ({100 * gbs32 / bw:.1f}% of stream), "
+            f"rel-L2 {m32[3]:.2e}, max-abs {m32[0]:.2e}")
+        result["fp32c"] = {
+            "value": round(glups32, 3), "unit": "GLUP/s", "dtype": "f32 (compensated, register carry)",
+            "kblock": t32.kblock, "tune": {"rows": t32.rows, "waves": t32.waves, "zchunk": t32.zchunk},
+            "ms_per_step": round(1e3 * wall32 / a.steps, 5),
+            "achieved_gbs": round(gbs32, 1), "frac_of_stream": round(gbs32 / bw, 4),
+            "algorithmic_bytes_per_lup": round(12.0 / t32.kblock, 3),
+            "speedup_vs_fp64": round(glups32 / glups, 3),
+            "rel_l2_vs_fp64": m32[3], "max_abs_vs_fp64": m32[0], "max_rel_vs_fp64": m32[4],
+            "targets": {"frac_of_stream>=0.8": gbs32 / bw >= 0.8,
+                        "speedup>=1.8": glups32 / glups >= 1.8,
+                        "rel_l2<=1e-5": m32[3] <= 1e-5},
+        }
+
+    # ---- CPU baseline ----
+    if not a.no_cpu:
+        log("CPU baseline ...")
+        result["cpu_baseline"] = cpu_baseline()
+        log(f"CPU: {result['cpu_baseline']['value']:.3f} GLUP/s "
+            f"({result['cpu_baseline']['cores']} threads)")
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

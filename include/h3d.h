@@ -1,0 +1,260 @@
+/*
+ * h3d.h -- C ABI of the MI355X-native 3-D variable-coefficient heat update.
+ *
+ * The reference (ezhilmathik/OpenMP-Offloading-Intranode-Artifact) has no
+ * plugin layer; its boundary is the kernel signature shared by its OpenMP,
+ * HIP and CUDA programs plus the `./bin n T` process contract.  The first
+ * group of entry points below reproduces that kernel signature exactly;
+ * the rest are the grid/stream forms the h3d host program and the tests use.
+ *
+ * Conventions (all entry points):
+ *  - Memory is owned by the caller.  Device pointers come from
+ *    h3d_target_alloc (or any hipMalloc'd memory on the same device).
+ *  - u_new != u_old; the kernels write interior points only, so the caller
+ *    seeds u_new's Dirichlet ring (1-hip.cpp:222-225 does the same).
+ *  - Layout [k][j][i], i fastest; point (k,j,i) at k*sz + j*sy + i.  The
+ *    dense reference layout is sy = nx, sz = nx*ny (1-openmp.c:28-29).
+ *  - Calls are ordered on the given HIP stream (void* so that this header
+ *    needs no HIP include; NULL = the device's default stream).  They are
+ *    re-entrant on distinct streams and never exit or abort.
+ *  - Return value: 0 on success, a negative H3D_E* code for a rejected
+ *    argument, or a positive hipError_t code from the runtime.
+ */
+#ifndef H3D_H
+#define H3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define H3D_OK 0
+#define H3D_EARG (-1)   /* invalid argument: null pointer, size, range  */
+#define H3D_ELIMIT (-2) /* supported limit exceeded (e.g. time block)   */
+#define H3D_ENODEV (-3) /* no such device                                */
+#define H3D_EHOST (-4)  /* host-side failure (malloc, file I/O)          */
+
+typedef struct h3d_grid {
+  int nx, ny, nz; /* points per axis, Dirichlet ring included          */
+  int sy;         /* row stride in elements, sy >= nx                  */
+  long long sz;   /* plane stride in elements, sz >= (long long)sy*ny  */
+} h3d_grid;
+
+/* Launch shape of the stencil kernels.  Zero fields take the library's
+ * defaults (or the autotuner's cached choice). */
+typedef struct h3d_tune {
+  int rows;   /* y rows per lane held in registers: 1, 2, 4 or 8     */
+  int waves;  /* wavefronts per workgroup, stacked in y: 1..16        */
+  int zchunk; /* planes marched by one workgroup (0 = whole range)    */
+  int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..8    */
+} h3d_tune;
+
+/* ---------------------------------------------------------------------- */
+/* Reference kernel signature, dense layout, device_id's default stream.  */
+
+/* Replaces heat_update (OpenMP_Single_Source/1-openmp.c:55-58;
+ * 3D_Heat_Benchmark/AMD-MI250X/Source-Results/HIP/1-hip.cpp:117-120):
+ * every interior point k = 1..nz-2.  rhs may be NULL (treated as zero). */
+int h3d_heat_update(double *u_new, const double *u_old, const double *rhs,
+                    const double *kappa, int nz, int ny, int nx,
+                    double factor, int device_id);
+
+/* Replaces heat_update_interior (OpenMP_Single_Source/2-openmp.c:214-219;
+ * 4-openmp-stream-omp-p2p.c:308-313): planes k_start..k_end inclusive. */
+int h3d_heat_update_interior(double *u_new, const double *u_old,
+                             const double *rhs, const double *kappa, int nz,
+                             int ny, int nx, double factor, int k_start,
+                             int k_end, int device_id);
+
+/* Replace heat_update_boundary_bottom / _top (2-openmp.c:145-148,
+ * 4-openmp-stream-omp-p2p.c:230-234 and :267-271): plane k = 1 / nz-2. */
+int h3d_heat_update_boundary_bottom(double *u_new, const double *u_old,
+                                    const double *rhs, const double *kappa,
+                                    int nz, int ny, int nx, double factor,
+                                    int device_id);
+int h3d_heat_update_boundary_top(double *u_new, const double *u_old,
+                                 const double *rhs, const double *kappa,
+                                 int nz, int ny, int nx, double factor,
+                                 int device_id);
+
+/* ---------------------------------------------------------------------- */
+/* Grid/stream forms.                                                      */
+
+/* One FP64 update of planes k_start..k_end (1 <= k_start, k_end <= nz-2)
+ * in the reference's association order, without FMA contraction: bit
+ * identical to heat3D_variable_coeff.c:174-181. tune may be NULL. */
+int h3d_step_f64(double *u_new, const double *u_old, const double *rhs,
+                 const double *kappa, const h3d_grid *g, double factor,
+                 int k_start, int k_end, const h3d_tune *tune, void *stream);
+
+/* `steps` FP64 updates with the u_a/u_b ping-pong of the reference's time
+ * loop (1-openmp.c:193-205).  graph != 0 records two steps in a hipGraph
+ * and replays it.  *ms receives the device time of the loop measured with
+ * HIP events on `stream`; *final_in_b is 1 if the result is in u_b. */
+int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
+                const double *kappa, const h3d_grid *g, double factor,
+                int steps, const h3d_tune *tune, int graph, void *stream,
+                float *ms, int *final_in_b);
+
+/* Face pack/unpack: the whole XY plane k including its x/y ring, written
+ * to/read from a dense [j][i] buffer of nx*ny values -- the order of the
+ * reference's plane transfer (2-openmp.c:321,
+ * 4-openmp-stream-omp-p2p.c:609-616).  Bit-exact copies. */
+int h3d_pack_face_f64(double *buf, const double *u, const h3d_grid *g, int k,
+                      void *stream);
+int h3d_unpack_face_f64(double *u, const double *buf, const h3d_grid *g,
+                        int k, void *stream);
+int h3d_pack_face_f32(float *buf, const float *u, const h3d_grid *g, int k,
+                      void *stream);
+int h3d_unpack_face_f32(float *u, const float *buf, const h3d_grid *g, int k,
+                        void *stream);
+
+/* FP64 diagnostics over the whole array (ring included), deterministic:
+ * out[0] = sum u (checksum), out[1] = sum u^2, out[2] = max |u|,
+ * out[3] = number of non-finite values.  Host `out`; synchronises stream. */
+int h3d_stats_f64(const double *u, const h3d_grid *g, double *out,
+                  void *stream);
+int h3d_stats_f32(const float *u, const h3d_grid *g, double *out,
+                  void *stream);
+
+/* Error metrics of compare.py:47-76 and the PASS rule of verify.sh:239-286,
+ * computed on the device in FP64 over the whole array:
+ * out[0] max_abs, [1] mean_abs, [2] rms, [3] rel_l2, [4] max_rel over
+ * |ref| > 1e-12, [5] argmax (flat k,j,i index of the first max |d|),
+ * [6] NaN count, [7] Inf count, [8] cells failing |d| <= atol + rtol|ref|
+ * or non-finite.  cand is FP64 (_f64) or FP32 (_f32). */
+int h3d_compare_f64(const double *ref, const double *cand, const h3d_grid *g,
+                    double atol, double rtol, double *out, void *stream);
+int h3d_compare_f32(const double *ref, const float *cand, const h3d_grid *g,
+                    double atol, double rtol, double *out, void *stream);
+
+/* max over interior cells of sum_{6 faces}(kappa_c + kappa_nb), in FP64;
+ * the explicit scheme keeps the discrete maximum principle iff
+ * dt <= 2 h^2 / result.  No reference counterpart (the reference fixes
+ * dt = h^2/(12*0.95), heat3D_variable_coeff.c:80-82). */
+int h3d_face_sum_max_f64(const double *kappa, const h3d_grid *g, double *out,
+                         void *stream);
+
+/* ---------------------------------------------------------------------- */
+/* FP32 storage paths (no reference counterpart; the reference is FP64).   */
+
+#define H3D_F32_PLAIN 0 /* u' = u + d                                      */
+#define H3D_F32_CARRY 1 /* compensated; residual carried in registers
+                           across the fused steps of one pass             */
+#define H3D_F32_COMP16 2 /* compensated; residual also kept between passes
+                           in an FP16 field scaled by the cell's ulp      */
+
+/* One HBM pass of `nsteps` (1..8) fused time steps over the interior of
+ * a grid stored in FP32.  kappa is exactly one of kappa_f32 / kappa_bf16
+ * (BF16 bits of kappa/scale; pass factor*scale as `factor`).  comp_in /
+ * comp_out (FP16 bits) are used in mode H3D_F32_COMP16 only.  Cell update:
+ * S in the FP64 kernel's order in FP32, d = factor*S, then TwoSum of u and
+ * d + e.  u_out's ring must hold the Dirichlet values. */
+int h3d_pass_f32(float *u_out, const float *u_in, const float *kappa_f32,
+                 const uint16_t *kappa_bf16, uint16_t *comp_out,
+                 const uint16_t *comp_in, const h3d_grid *g, float factor,
+                 int nsteps, int mode, const h3d_tune *tune, void *stream);
+
+/* `steps` FP32 updates as passes of tune->kblock steps (remainder last),
+ * ping-ponging u_a/u_b (and c_a/c_b in COMP16 mode).  graph != 0 replays a
+ * captured pair of passes.  *ms: loop time by HIP events on `stream`;
+ * *final_in_b: 1 if the result (and residual field) ended in u_b (c_b). */
+int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
+                const uint16_t *kappa_bf16, uint16_t *c_a, uint16_t *c_b,
+                const h3d_grid *g, float factor, int steps, int mode,
+                const h3d_tune *tune, int graph, void *stream, float *ms,
+                int *final_in_b);
+
+/* Choose tune fields for a problem by timing candidate shapes on the
+ * device (a few passes each, on scratch copies of the caller's buffers);
+ * results are cached per (kernel, nx, ny, nz, mode, kappa format).
+ * kind: 0 = FP64 step, 1 = FP32 pass (kblock chosen too unless
+ * fixed_kblock > 0).  The buffers' contents are not modified. */
+int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
+                 int kappa_bf16, const h3d_grid *g, int mode,
+                 int fixed_kblock, void *stream, h3d_tune *out);
+
+/* ---------------------------------------------------------------------- */
+/* Runtime helpers (stand-ins for omp_target_alloc / omp_target_memcpy of
+ * 1-openmp.c:166-179,214).                                                */
+
+int h3d_device_count(int *count);
+int h3d_target_alloc(void **ptr, size_t bytes, int device_id);
+int h3d_target_free(void *ptr, int device_id);
+/* Synchronous copy; dst_device/src_device = -1 for host memory. */
+int h3d_target_memcpy(void *dst, const void *src, size_t bytes,
+                      int dst_device, int src_device);
+int h3d_device_synchronize(int device_id);
+/* A non-blocking HIP stream on device_id (default priority), returned as
+ * void*; h3d_stream_synchronize waits for all work queued on it. */
+int h3d_stream_create(void **stream, int device_id);
+int h3d_stream_destroy(void *stream);
+int h3d_stream_synchronize(void *stream);
+const char *h3d_error_string(int code);
+/* Name of the GPU architecture of device_id (e.g. "gfx950") into buf. */
+int h3d_device_arch(int device_id, char *buf, int buflen);
+
+/* HBM bandwidth probe: device-to-device copy and triad (a = b + s*c) over
+ * `bytes` per array, best of `reps` timed launches, in GB/s (1e9 B/s). */
+int h3d_stream_probe(size_t bytes, int reps, int device_id, double *copy_gbs,
+                     double *triad_gbs);
+
+/* ---------------------------------------------------------------------- */
+/* Host-side problem set-up (plain C; no device involved).                 */
+
+/* heat3D_variable_coeff.c:79-84: h = 1/(n-1), dt = h^2/(12*0.95),
+ * factor = dt/h/h/2. */
+int h3d_params_reference(int n, double *h, double *dt, double *factor);
+/* heat3D_variable_coeff.c:161-193: number of passes of
+ * `while (t < T) { ...; t += dt; }`. */
+int h3d_count_steps(double T, double dt);
+/* heat3D_variable_coeff.c:110-150 / 1-openmp.c:153-163 on a grid with
+ * nx = ny = nz = n (strides honoured).  rhs may be NULL. */
+int h3d_init_reference(double *u, double *rhs, double *kappa,
+                       const h3d_grid *g);
+/* 4-openmp-stream-omp-p2p.c:208-223 (init_partition), dense slab. */
+int h3d_init_partition(double *u, double *rhs, double *kappa, int nz_local,
+                       int gk_start, int ng, int n, int ny, int nx, double h,
+                       double dt);
+/* 4-openmp-stream-omp-p2p.c:391-415 (parts = 4) / 2-openmp.c:294-316
+ * (parts = 2): nz_d[p] planes incl. 2 ghosts, first owned global plane
+ * gk_start[p].  H3D_EARG if a slab has fewer than 4 planes. */
+int h3d_slab_decompose(int n, int parts, int *nz_d, int *gk_start);
+
+/* Seeded inputs (no reference counterpart: the reference has no RNG).
+ * Interior m^3 of a global grid of (m+2)^3 points, h = 1/(m+1), x = i*h.
+ *   kappa_kind 0: kappa = (1 + 0.5 sin2pix sin2piy sin2piz) * U[0.9,1.1]
+ *   kappa_kind 1: kappa = exp(0.5 Z), Z ~ N(0,1) (Box-Muller)
+ *   u0 = exp(-|x-0.5|^2 / (2*0.1^2)) + U[0,1e-3] inside, 0 on the ring.
+ * Random numbers are a counter-based hash of (seed, member, stream,
+ * global linear index), so slabs and tiles of one problem generate the
+ * same values.  This fills a block of nz planes starting at global plane
+ * gk0, of a global problem whose axes are gnx, gny, gnz points (the block
+ * covers all of x and y: g->nx == gnx, g->ny == gny).  Either output may
+ * be NULL. */
+int h3d_gen_seeded(double *u0, double *kappa, const h3d_grid *g, int gk0,
+                   int gnx, int gny, int gnz, int kappa_kind, uint64_t seed,
+                   int member);
+/* The generator's uniform variate in [0,1) (exposed for testing). */
+double h3d_rng_uniform(uint64_t seed, int member, int stream, uint64_t idx);
+
+/* Host FP64 max over interior of sum_{6 faces}(kappa_c + kappa_nb). */
+double h3d_face_sum_max_host(const double *kappa, const h3d_grid *g);
+
+/* BF16 storage of kappa with a per-field power-of-two scale s: writes
+ * bf16(kappa/s) (round to nearest even) and s; kappa is then represented
+ * exactly by bf16_to_f32(bits)*s (use it, not the original, as the FP64
+ * reference input).  n values. */
+int h3d_quantize_bf16(const double *kappa, uint16_t *bits, size_t n,
+                      double *scale);
+
+/* `%.17g` one value per line, k,j,i order, all nz*ny*nx points
+ * (1-openmp.c:41-51). */
+int h3d_write_dump(const char *fname, const double *u, const h3d_grid *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* H3D_H */

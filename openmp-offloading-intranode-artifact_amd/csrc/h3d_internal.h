@@ -1,0 +1,79 @@
+// Shared device helpers and argument checks of the h3d HIP library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "h3d.h"
+
+#define H3D_TRY(expr)                                  \
+  do {                                                 \
+    hipError_t h3d_e_ = (expr);                        \
+    if (h3d_e_ != hipSuccess) return (int)h3d_e_;      \
+  } while (0)
+
+// Status of the last launch on this thread, as an h3d return code.
+static inline int h3d_launch_status() {
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? H3D_OK : (int)e;
+}
+
+static inline hipStream_t h3d_stream(void *s) {
+  return reinterpret_cast<hipStream_t>(s);
+}
+
+// A grid the kernels can address: positive sizes, strides that hold a row
+// and a plane, and at least one interior point per axis.
+static inline bool h3d_grid_ok(const h3d_grid *g) {
+  if (!g) return false;
+  if (g->nx < 3 || g->ny < 3 || g->nz < 3) return false;
+  if (g->sy < g->nx) return false;
+  if (g->sz < (long long)g->sy * g->ny) return false;
+  // within-plane offsets are 32-bit in the kernels
+  if (g->sz > 0x7fffffffLL) return false;
+  return true;
+}
+
+static inline size_t h3d_grid_elems(const h3d_grid *g) {
+  return (size_t)g->sz * (size_t)g->nz;
+}
+
+// XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs
+// (workgroup b runs on XCD b % 8), each with its own L2.  Remapping so
+// that XCD x receives a contiguous run of logical tiles keeps neighbouring
+// tiles, which share halo rows, in one L2.
+__device__ __forceinline__ unsigned h3d_xcd_tile(unsigned b, unsigned nb) {
+  const unsigned nxcd = 8;
+  if (nb % nxcd) return b;
+  return (b % nxcd) * (nb / nxcd) + b / nxcd;
+}
+
+// Whole-wavefront lane shifts through DPP (wave_shr:1 / wave_shl:1): lane
+// l receives lane l-1 (shr) or l+1 (shl); the lane with no source keeps
+// `edge`.  64-bit values move as two dwords.
+__device__ __forceinline__ double h3d_from_left(double v, double edge) {
+  int2 s = *reinterpret_cast<int2 *>(&v), o = *reinterpret_cast<int2 *>(&edge), r;
+  r.x = __builtin_amdgcn_update_dpp(o.x, s.x, 0x138, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp(o.y, s.y, 0x138, 0xf, 0xf, false);
+  return *reinterpret_cast<double *>(&r);
+}
+__device__ __forceinline__ double h3d_from_right(double v, double edge) {
+  int2 s = *reinterpret_cast<int2 *>(&v), o = *reinterpret_cast<int2 *>(&edge), r;
+  r.x = __builtin_amdgcn_update_dpp(o.x, s.x, 0x130, 0xf, 0xf, false);
+  r.y = __builtin_amdgcn_update_dpp(o.y, s.y, 0x130, 0xf, 0xf, false);
+  return *reinterpret_cast<double *>(&r);
+}
+__device__ __forceinline__ float h3d_from_left(float v, float edge) {
+  int r = __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v),
+                                      0x138, 0xf, 0xf, false);
+  return __int_as_float(r);
+}
+__device__ __forceinline__ float h3d_from_right(float v, float edge) {
+  int r = __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v),
+                                      0x130, 0xf, 0xf, false);
+  return __int_as_float(r);
+}
+
+// Library-internal entry points shared between translation units.
+int h3d_default_tune_f64(const h3d_grid *g, int k_start, int k_end,
+                         h3d_tune *t);

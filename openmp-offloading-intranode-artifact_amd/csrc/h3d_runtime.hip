@@ -1,0 +1,214 @@
+// Device memory, copies, streams and the HBM bandwidth probe.
+//
+// The memory helpers stand where the reference calls omp_target_alloc /
+// omp_target_memcpy / omp_target_free (OpenMP_Single_Source/1-openmp.c:
+// 166-179, 214, 231-232) and hipMalloc / hipMemcpy in its HIP twin
+// (HIP/1-hip.cpp:217-225).  Unlike the reference they return a status
+// instead of exiting.
+#include <string.h>
+
+#include "h3d_internal.h"
+
+extern "C" int h3d_device_count(int *count) {
+  if (!count) return H3D_EARG;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e == hipErrorNoDevice) n = 0;
+  else if (e != hipSuccess) return (int)e;
+  *count = n;
+  return H3D_OK;
+}
+
+static int set_dev(int device_id) {
+  int n = 0;
+  int rc = h3d_device_count(&n);
+  if (rc != H3D_OK) return rc;
+  if (device_id < 0 || device_id >= n) return H3D_ENODEV;
+  H3D_TRY(hipSetDevice(device_id));
+  return H3D_OK;
+}
+
+extern "C" int h3d_target_alloc(void **ptr, size_t bytes, int device_id) {
+  if (!ptr) return H3D_EARG;
+  *ptr = nullptr;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  if (bytes == 0) return H3D_OK;
+  H3D_TRY(hipMalloc(ptr, bytes));
+  return H3D_OK;
+}
+
+extern "C" int h3d_target_free(void *ptr, int device_id) {
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  if (ptr) H3D_TRY(hipFree(ptr));
+  return H3D_OK;
+}
+
+extern "C" int h3d_target_memcpy(void *dst, const void *src, size_t bytes,
+                                 int dst_device, int src_device) {
+  if ((!dst || !src) && bytes) return H3D_EARG;
+  if (bytes == 0) return H3D_OK;
+  hipMemcpyKind kind;
+  if (dst_device < 0 && src_device < 0) {
+    memcpy(dst, src, bytes);
+    return H3D_OK;
+  } else if (dst_device < 0) {
+    kind = hipMemcpyDeviceToHost;
+  } else if (src_device < 0) {
+    kind = hipMemcpyHostToDevice;
+  } else {
+    kind = hipMemcpyDeviceToDevice;
+  }
+  int rc = set_dev(dst_device >= 0 ? dst_device : src_device);
+  if (rc != H3D_OK) return rc;
+  if (kind == hipMemcpyDeviceToDevice && dst_device != src_device) {
+    H3D_TRY(hipMemcpyPeer(dst, dst_device, src, src_device, bytes));
+    return H3D_OK;
+  }
+  H3D_TRY(hipMemcpy(dst, src, bytes, kind));
+  return H3D_OK;
+}
+
+extern "C" int h3d_device_synchronize(int device_id) {
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  H3D_TRY(hipDeviceSynchronize());
+  return H3D_OK;
+}
+
+extern "C" int h3d_device_arch(int device_id, char *buf, int buflen) {
+  if (!buf || buflen < 1) return H3D_EARG;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  hipDeviceProp_t p;
+  H3D_TRY(hipGetDeviceProperties(&p, device_id));
+  strncpy(buf, p.gcnArchName, (size_t)buflen - 1);
+  buf[buflen - 1] = 0;
+  return H3D_OK;
+}
+
+extern "C" int h3d_stream_create(void **stream, int device_id) {
+  if (!stream) return H3D_EARG;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  hipStream_t s;
+  H3D_TRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  *stream = s;
+  return H3D_OK;
+}
+
+extern "C" int h3d_stream_destroy(void *stream) {
+  if (!stream) return H3D_EARG;
+  H3D_TRY(hipStreamDestroy(h3d_stream(stream)));
+  return H3D_OK;
+}
+
+extern "C" int h3d_stream_synchronize(void *stream) {
+  H3D_TRY(hipStreamSynchronize(h3d_stream(stream)));
+  return H3D_OK;
+}
+
+extern "C" const char *h3d_error_string(int code) {
+  switch (code) {
+    case H3D_OK: return "success";
+    case H3D_EARG: return "invalid argument";
+    case H3D_ELIMIT: return "supported limit exceeded";
+    case H3D_ENODEV: return "no such device";
+    case H3D_EHOST: return "host-side failure";
+    default:
+      if (code > 0) return hipGetErrorString((hipError_t)code);
+      return "unknown error";
+  }
+}
+
+// ---------------------------------------------------------------------------
+// HBM bandwidth probe.  Arrays far larger than the 256 MB Infinity Cache;
+// 16-byte vector accesses, grid-stride, nontemporal stores.
+
+namespace {
+
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__global__ void __launch_bounds__(256)
+probe_copy(v2d *__restrict__ a, const v2d *__restrict__ b, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    __builtin_nontemporal_store(b[t], &a[t]);
+}
+
+__global__ void __launch_bounds__(256)
+probe_triad(v2d *__restrict__ a, const v2d *__restrict__ b,
+            const v2d *__restrict__ c, double s, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) {
+    const v2d r = b[t] + s * c[t];
+    __builtin_nontemporal_store(r, &a[t]);
+  }
+}
+
+__global__ void probe_fill(v2d *a, size_t n, double v) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    a[t] = v2d{v, v};
+}
+
+}  // namespace
+
+extern "C" int h3d_stream_probe(size_t bytes, int reps, int device_id,
+                                double *copy_gbs, double *triad_gbs) {
+  if (bytes < (1u << 20) || reps < 1) return H3D_EARG;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  const size_t n = bytes / sizeof(v2d);
+  v2d *a = nullptr, *b = nullptr, *c = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  double best_copy = 0, best_triad = 0;
+  hipError_t e = hipMalloc(&a, n * sizeof(v2d));
+  if (e == hipSuccess) e = hipMalloc(&b, n * sizeof(v2d));
+  if (e == hipSuccess) e = hipMalloc(&c, n * sizeof(v2d));
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  int dev_cus = 256;
+  if (e == hipSuccess) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device_id) == hipSuccess) dev_cus = p.multiProcessorCount;
+  }
+  const unsigned blocks = (unsigned)(dev_cus * 8);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(probe_fill, dim3(blocks), dim3(256), 0, 0, b, n, 1.0);
+    hipLaunchKernelGGL(probe_fill, dim3(blocks), dim3(256), 0, 0, c, n, 2.0);
+    e = hipGetLastError();
+  }
+  for (int r = 0; e == hipSuccess && r <= reps; ++r) {
+    float ms = 0;
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(probe_copy, dim3(blocks), dim3(256), 0, 0, a, b, n);
+    (void)hipEventRecord(e1, 0);
+    e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && r > 0) {  // first launch is a warm-up
+      double gbs = 2.0 * n * sizeof(v2d) / (ms * 1e-3) / 1e9;
+      if (gbs > best_copy) best_copy = gbs;
+    }
+    if (e != hipSuccess) break;
+    (void)hipEventRecord(e0, 0);
+    hipLaunchKernelGGL(probe_triad, dim3(blocks), dim3(256), 0, 0, a, b, c, 3.0, n);
+    (void)hipEventRecord(e1, 0);
+    e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && r > 0) {
+      double gbs = 3.0 * n * sizeof(v2d) / (ms * 1e-3) / 1e9;
+      if (gbs > best_triad) best_triad = gbs;
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  if (c) (void)hipFree(c);
+  if (e != hipSuccess) return (int)e;
+  if (copy_gbs) *copy_gbs = best_copy;
+  if (triad_gbs) *triad_gbs = best_triad;
+  return H3D_OK;
+}

@@ -1,0 +1,150 @@
+// Autotuner: times candidate launch shapes of the FP64 step or the FP32
+// pass on the caller's problem and keeps the fastest per problem shape.
+//
+// Candidates are timed with HIP events on the caller's stream, writing
+// into scratch output buffers (the caller's fields are only read), best
+// of three launches after one warm-up.  For the FP32 pass the figure of
+// merit is time per time step (pass time / kblock).
+#include <mutex>
+#include <vector>
+
+#include "h3d_internal.h"
+
+int h3d_default_tune_f32(const h3d_grid *g, int kblock, h3d_tune *t);
+
+namespace {
+
+struct Key {
+  int kind, nx, ny, nz, sy, mode, bf16, fixed_kb;
+  long long sz;
+  bool operator==(const Key &o) const {
+    return kind == o.kind && nx == o.nx && ny == o.ny && nz == o.nz &&
+           sy == o.sy && sz == o.sz && mode == o.mode && bf16 == o.bf16 &&
+           fixed_kb == o.fixed_kb;
+  }
+};
+
+std::mutex cache_mu;
+std::vector<std::pair<Key, h3d_tune>> cache;
+
+template <typename F>
+int time_best(hipStream_t s, F launch, float *best_ms) {
+  hipEvent_t e0, e1;
+  H3D_TRY(hipEventCreate(&e0));
+  H3D_TRY(hipEventCreate(&e1));
+  int rc = launch();  // warm-up (also rejects invalid shapes)
+  float best = 1e30f;
+  for (int r = 0; rc == H3D_OK && r < 3; ++r) {
+    float ms = 0;
+    hipError_t e = hipEventRecord(e0, s);
+    if (e == hipSuccess) rc = launch();
+    if (rc != H3D_OK) break;
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e != hipSuccess) { rc = (int)e; break; }
+    if (ms < best) best = ms;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *best_ms = best;
+  return rc;
+}
+
+}  // namespace
+
+extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
+                            int kappa_bf16, const h3d_grid *g, int mode,
+                            int fixed_kblock, void *stream, h3d_tune *out) {
+  (void)u_b;
+  if (!u_a || !kappa || !out || !h3d_grid_ok(g)) return H3D_EARG;
+  if (kind != 0 && kind != 1) return H3D_EARG;
+  if (kind == 1 && (mode < 0 || mode > 2)) return H3D_EARG;
+  if (fixed_kblock < 0 || fixed_kblock > 8) return H3D_ELIMIT;
+  const Key key{kind, g->nx, g->ny, g->nz, g->sy, kind ? mode : 0,
+                kind ? kappa_bf16 : 0, kind ? fixed_kblock : 0, g->sz};
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    for (auto &e : cache)
+      if (e.first == key) { *out = e.second; return H3D_OK; }
+  }
+  hipStream_t s = h3d_stream(stream);
+  const size_t cells = h3d_grid_elems(g);
+  const size_t esz = kind == 0 ? sizeof(double) : sizeof(float);
+  void *scratch = nullptr, *c_in = nullptr, *c_out = nullptr;
+  H3D_TRY(hipMalloc(&scratch, cells * esz));
+  hipError_t e = hipSuccess;
+  if (kind == 1 && mode == 2) {
+    e = hipMalloc(&c_in, cells * 2);
+    if (e == hipSuccess) e = hipMalloc(&c_out, cells * 2);
+    if (e == hipSuccess) e = hipMemsetAsync(c_in, 0, cells * 2, s);
+  }
+  h3d_tune best{};
+  float best_ms = 1e30f;
+  int rc = (e == hipSuccess) ? H3D_OK : (int)e;
+  if (rc == H3D_OK && kind == 0) {
+    const int planes = g->nz - 2;
+    const int zcs[] = {0, 256, 128, 64, 32};
+    for (int rows : {1, 2, 4})
+      for (int waves : {2, 4, 8})
+        for (int zc : zcs) {
+          if (zc > planes) continue;
+          h3d_tune t{rows, waves, zc, 1};
+          float ms;
+          int r = time_best(s, [&] {
+            return h3d_step_f64((double *)scratch, (const double *)u_a, nullptr,
+                                (const double *)kappa, g, 1e-3, 1, g->nz - 2,
+                                &t, s);
+          }, &ms);
+          if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
+          if (r == H3D_OK && ms < best_ms) {
+            best_ms = ms;
+            best = t;
+            if (best.zchunk == 0) {
+              h3d_tune d;
+              h3d_default_tune_f64(g, 1, g->nz - 2, &d);
+              best.zchunk = d.zchunk;
+            }
+          }
+        }
+  } else if (rc == H3D_OK) {
+    const float *kf = kappa_bf16 ? nullptr : (const float *)kappa;
+    const uint16_t *kb = kappa_bf16 ? (const uint16_t *)kappa : nullptr;
+    for (int kbk = 1; kbk <= 4 && rc == H3D_OK; ++kbk) {
+      if (fixed_kblock && kbk != fixed_kblock) continue;
+      for (int rows : {2, 4})
+        for (int waves : {4, 8})
+          for (int zc : {0, 64, 128}) {
+            h3d_tune t{rows, waves, zc, kbk};
+            float ms;
+            int r = time_best(s, [&] {
+              return h3d_pass_f32((float *)scratch, (const float *)u_a, kf, kb,
+                                  (uint16_t *)c_out, (const uint16_t *)c_in, g,
+                                  1e-3f, kbk, mode, &t, s);
+            }, &ms);
+            if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
+            if (r == H3D_OK && ms / kbk < best_ms) {
+              best_ms = ms / kbk;
+              best = t;
+              if (best.zchunk == 0) {
+                h3d_tune d;
+                h3d_default_tune_f32(g, kbk, &d);
+                best.zchunk = d.zchunk;
+              }
+            }
+          }
+    }
+  }
+  (void)hipStreamSynchronize(s);
+  (void)hipFree(scratch);
+  if (c_in) (void)hipFree(c_in);
+  if (c_out) (void)hipFree(c_out);
+  if (rc != H3D_OK) return rc;
+  if (best_ms >= 1e30f) return H3D_ELIMIT;  // no candidate ran
+  {
+    std::lock_guard<std::mutex> lk(cache_mu);
+    cache.push_back({key, best});
+  }
+  *out = best;
+  return H3D_OK;
+}

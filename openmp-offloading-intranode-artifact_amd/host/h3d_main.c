@@ -1,0 +1,461 @@
+/*
+ * h3d -- host program of the MI355X-native 3-D variable-coefficient heat
+ * update.  Drop-in for the reference's `./bin n T` binaries
+ * (OpenMP_Single_Source/1-openmp.c:87-241, HIP/1-hip.cpp:137-293):
+ *
+ *   h3d [n [T]] [options]
+ *
+ * n counts points per axis including the Dirichlet ring (default 51) and T
+ * is the final time (default 1.0); the step count is the number of passes
+ * of `while (t < T) t += dt` (heat3D_variable_coeff.c:161-193).  stdout
+ * carries the lines the reference's pipeline parses ("Estimated
+ * timesteps: ~N", "N timesteps complete", "Time = S seconds",
+ * aggregate.sh:31-40) and WRITE_TXT selects the %.17g solution dump
+ * (default on, as in 1-openmp.c:31-39), here u_h3d_1gpu.txt.
+ *
+ * Options (all new; the reference has none):
+ *   --steps S          fixed step count instead of T
+ *   --mode reference|seeded   reference fields (default) or seeded fields
+ *   --kappa sine|lognormal    seeded kappa generator (default sine)
+ *   --seed N           seed of the seeded generators (default 20260101)
+ *   --precision f64|f32|f32c|f32c16
+ *                      FP64 plain (default); FP32 storage plain; FP32
+ *                      compensated with register carry; with FP16 residual
+ *   --kappa-bf16       store kappa in BF16 with a per-field scale (FP32)
+ *   --kblock K         FP32: time steps fused per HBM pass (1..8)
+ *   --tile R,W,Z       rows per lane, waves per workgroup, planes per chunk
+ *   --autotune         time candidate launch shapes first and use the best
+ *   --graph            replay the step loop from a hipGraph
+ *   --slab P:p         run slab p of a P-way z split (P = 2 or 4) with its
+ *                      two ghost planes refreshed every step by in-HBM face
+ *                      pack/unpack from the slab's own edge planes
+ *   --check            FP32 runs: also run the FP64 kernel on the same
+ *                      inputs and report rel-L2 / max errors against it
+ *   --device D         HIP device (default 0)
+ * Exit status: 0 on success, 1 on any error (message on stderr).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <strings.h>
+#include <time.h>
+#include <unistd.h>
+
+#include "h3d.h"
+
+
+#define CHECK(call)                                                        \
+  do {                                                                     \
+    int rc_ = (call);                                                      \
+    if (rc_ != H3D_OK) {                                                   \
+      fprintf(stderr, "ERROR: %s failed at %s:%d: %s (%d)\n", #call,       \
+              __FILE__, __LINE__, h3d_error_string(rc_), rc_);             \
+      exit(1);                                                             \
+    }                                                                      \
+  } while (0)
+
+static int should_write_txt(void)
+{
+  const char *e = getenv("WRITE_TXT");
+  if (!e) return 1;
+  if (!strcmp(e, "0")) return 0;
+  if (!strcasecmp(e, "false")) return 0;
+  if (!strcasecmp(e, "no")) return 0;
+  return 1;
+}
+
+static double wtime(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void *xmalloc(size_t bytes)
+{
+  void *p = malloc(bytes ? bytes : 1);
+  if (!p) {
+    fprintf(stderr, "ERROR: malloc failed (host arrays, %zu bytes)\n", bytes);
+    exit(1);
+  }
+  return p;
+}
+
+static void usage(void)
+{
+  fprintf(stderr,
+          "usage: h3d [n [T]] [--steps S] [--mode reference|seeded] "
+          "[--kappa sine|lognormal] [--seed N] [--precision f64|f32|f32c|f32c16] "
+          "[--kappa-bf16] [--kblock K] [--tile R,W,Z] [--autotune] [--graph] "
+          "[--slab P:p] [--check] [--device D]\n");
+  exit(1);
+}
+
+typedef struct opts {
+  int n;
+  double T;
+  int steps; /* -1: from T */
+  int seeded, kappa_kind;
+  uint64_t seed;
+  int prec; /* 0 f64, 1 f32 plain, 2 f32c carry, 3 f32c fp16 */
+  int bf16, autotune, graph, check, device;
+  int slab_parts, slab_index;
+  h3d_tune tune;
+} opts;
+
+static opts parse(int argc, char **argv)
+{
+  opts o;
+  memset(&o, 0, sizeof o);
+  o.n = 51;
+  o.T = 1.0;
+  o.steps = -1;
+  o.seed = 20260101ULL;
+  int pos = 0;
+  for (int a = 1; a < argc; a++) {
+    const char *s = argv[a];
+    if (!strncmp(s, "--", 2)) {
+      const char *v = (a + 1 < argc) ? argv[a + 1] : NULL;
+      if (!strcmp(s, "--steps") && v) { o.steps = atoi(v); a++; }
+      else if (!strcmp(s, "--mode") && v) {
+        if (!strcmp(v, "reference")) o.seeded = 0;
+        else if (!strcmp(v, "seeded")) o.seeded = 1;
+        else usage();
+        a++;
+      } else if (!strcmp(s, "--kappa") && v) {
+        if (!strcmp(v, "sine")) o.kappa_kind = 0;
+        else if (!strcmp(v, "lognormal")) o.kappa_kind = 1;
+        else usage();
+        a++;
+      } else if (!strcmp(s, "--seed") && v) { o.seed = strtoull(v, NULL, 10); a++; }
+      else if (!strcmp(s, "--precision") && v) {
+        if (!strcmp(v, "f64")) o.prec = 0;
+        else if (!strcmp(v, "f32")) o.prec = 1;
+        else if (!strcmp(v, "f32c")) o.prec = 2;
+        else if (!strcmp(v, "f32c16")) o.prec = 3;
+        else usage();
+        a++;
+      } else if (!strcmp(s, "--kappa-bf16")) o.bf16 = 1;
+      else if (!strcmp(s, "--kblock") && v) { o.tune.kblock = atoi(v); a++; }
+      else if (!strcmp(s, "--tile") && v) {
+        if (sscanf(v, "%d,%d,%d", &o.tune.rows, &o.tune.waves, &o.tune.zchunk) < 2) usage();
+        a++;
+      } else if (!strcmp(s, "--autotune")) o.autotune = 1;
+      else if (!strcmp(s, "--graph")) o.graph = 1;
+      else if (!strcmp(s, "--check")) o.check = 1;
+      else if (!strcmp(s, "--slab") && v) {
+        if (sscanf(v, "%d:%d", &o.slab_parts, &o.slab_index) != 2) usage();
+        a++;
+      } else if (!strcmp(s, "--device") && v) { o.device = atoi(v); a++; }
+      else usage();
+    } else if (pos == 0) { o.n = atoi(s); pos++; }
+    else if (pos == 1) { o.T = atof(s); pos++; }
+    else usage();
+  }
+  if (o.n < 3) {
+    fprintf(stderr, "ERROR: n=%d: need at least 3 points per axis\n", o.n);
+    exit(1);
+  }
+  if (o.bf16 && o.prec == 0) {
+    fprintf(stderr, "ERROR: --kappa-bf16 needs an FP32 precision\n");
+    exit(1);
+  }
+  if (o.slab_parts && o.prec != 0) {
+    fprintf(stderr, "ERROR: --slab runs the FP64 kernel only\n");
+    exit(1);
+  }
+  return o;
+}
+
+/* FP32 copy of an FP64 field. */
+static float *to_f32(const double *src, size_t n)
+{
+  float *d = (float *)xmalloc(n * sizeof(float));
+  for (size_t c = 0; c < n; c++) d[c] = (float)src[c];
+  return d;
+}
+
+int main(int argc, char **argv)
+{
+  setvbuf(stdout, NULL, _IOLBF, 0);
+  opts o = parse(argc, argv);
+  const int n = o.n;
+  printf("\n**** ONE GPU HIP (h3d, MI355X) STARTS ****\n");
+
+  /* ---- slab geometry -------------------------------------------------- */
+  int nz = n, gk0 = 0; /* planes held and first held global plane */
+  if (o.slab_parts) {
+    int nz_d[4], gk_start[4];
+    if (o.slab_index < 0 || o.slab_index >= o.slab_parts ||
+        h3d_slab_decompose(n, o.slab_parts, nz_d, gk_start) != H3D_OK) {
+      fprintf(stderr, "ERROR: n=%d cannot be split %d ways (each slab needs >= 4 planes)\n",
+              n, o.slab_parts);
+      return 1;
+    }
+    printf("Block sizes:");
+    for (int p = 0; p < o.slab_parts; p++) printf(" nz%d=%d", p, nz_d[p]);
+    printf("\n");
+    nz = nz_d[o.slab_index];
+    gk0 = gk_start[o.slab_index] - 1; /* ghost plane 0 is global gk_start-1 */
+  }
+  const h3d_grid g = {n, n, nz, n, (long long)n * n};
+  const size_t cells = (size_t)nz * n * n;
+  const size_t total_points = (size_t)n * n * n;
+
+  printf("\n=== MEMORY ESTIMATE (GLOBAL) ===\n");
+  printf("Grid size: %d x %d x %d = %zu points\n", n, n, n, total_points);
+  printf("Memory per global array: %.3f GB\n", total_points * 8 / 1e9);
+  printf("================================\n\n");
+
+  /* ---- fields and parameters ----------------------------------------- */
+  double *u0 = (double *)xmalloc(cells * sizeof(double));
+  double *kap = (double *)xmalloc(cells * sizeof(double));
+  double *rhs = NULL;
+  double h, dt, factor, kappa_max;
+  if (!o.seeded) {
+    CHECK(h3d_params_reference(n, &h, &dt, &factor));
+    kappa_max = 0.95;
+    rhs = (double *)xmalloc(cells * sizeof(double));
+    if (o.slab_parts)
+      CHECK(h3d_init_partition(u0, rhs, kap, nz, gk0 + 1, 1, n, n, n, h, dt));
+    else
+      CHECK(h3d_init_reference(u0, rhs, kap, &g));
+  } else {
+    CHECK(h3d_gen_seeded(u0, kap, &g, gk0, n, n, n, o.kappa_kind, o.seed, 0));
+    h = 1.0 / (n - 1);
+    double fmax;
+    if (o.slab_parts) {
+      /* the bound must be global: every slab of the split uses one dt */
+      double *kg = (double *)xmalloc(total_points * sizeof(double));
+      const h3d_grid gg = {n, n, n, n, (long long)n * n};
+      CHECK(h3d_gen_seeded(NULL, kg, &gg, 0, n, n, n, o.kappa_kind, o.seed, 0));
+      fmax = h3d_face_sum_max_host(kg, &gg);
+      free(kg);
+    } else {
+      fmax = h3d_face_sum_max_host(kap, &g);
+    }
+    dt = 0.9 * 2.0 * h * h / fmax;
+    factor = dt / h / h / 2.0;
+    kappa_max = fmax / 12.0;
+  }
+
+  int steps = o.steps >= 0 ? o.steps : h3d_count_steps(o.T, dt);
+  double T = o.steps >= 0 ? steps * dt : o.T;
+  printf("=== SIMULATION PARAMETERS ===\n");
+  printf("Grid size (n):           %d\n", n);
+  printf("Final time (T):          %.6f\n", T);
+  printf("Grid spacing (h):        %.6e\n", h);
+  printf("Time step (dt):          %.6e\n", dt);
+  printf("Kappa_max:               %.6f\n", kappa_max);
+  printf("Diffusion factor:        %.6e\n", factor);
+  printf("Estimated timesteps:     ~%d\n", o.steps >= 0 ? steps : (int)(T / dt));
+  printf("CFL condition (dt/h^2):  %.6f (should be < 0.5)\n", dt / (h * h));
+  printf("Mode:                    %s%s, precision %s%s, seed %llu\n",
+         o.seeded ? "seeded" : "reference",
+         o.seeded ? (o.kappa_kind ? " (lognormal kappa)" : " (sine kappa)") : "",
+         (const char *[]){"f64", "f32", "f32c", "f32c16"}[o.prec],
+         o.bf16 ? " + bf16 kappa" : "", (unsigned long long)o.seed);
+  printf("==============================\n\n");
+
+  int ndev = 0;
+  CHECK(h3d_device_count(&ndev));
+  if (ndev < 1 || o.device >= ndev) {
+    fprintf(stderr, "ERROR: this version needs 1 HIP device, %d present\n", ndev);
+    return 1;
+  }
+  char arch[64] = "";
+  CHECK(h3d_device_arch(o.device, arch, sizeof arch));
+  printf("Using HIP device %d of %d (%s)\n", o.device, ndev, arch);
+  void *stream = NULL;
+  CHECK(h3d_stream_create(&stream, o.device));
+
+  /* ---- device buffers ------------------------------------------------- */
+  const int f32 = o.prec != 0;
+  const size_t esz = f32 ? sizeof(float) : sizeof(double);
+  void *d_ua, *d_ub, *d_kap = NULL, *d_rhs = NULL, *d_ca = NULL, *d_cb = NULL;
+  CHECK(h3d_target_alloc(&d_ua, cells * esz, o.device));
+  CHECK(h3d_target_alloc(&d_ub, cells * esz, o.device));
+  double kscale = 1.0;
+  float *h_u32 = NULL;
+  if (!f32) {
+    CHECK(h3d_target_alloc(&d_kap, cells * 8, o.device));
+    CHECK(h3d_target_memcpy(d_ua, u0, cells * 8, o.device, -1));
+    CHECK(h3d_target_memcpy(d_ub, u0, cells * 8, o.device, -1));
+    CHECK(h3d_target_memcpy(d_kap, kap, cells * 8, o.device, -1));
+    if (rhs) {
+      CHECK(h3d_target_alloc(&d_rhs, cells * 8, o.device));
+      CHECK(h3d_target_memcpy(d_rhs, rhs, cells * 8, o.device, -1));
+    }
+  } else {
+    if (rhs) {
+      fprintf(stderr, "ERROR: FP32 precisions run seeded mode only (--mode seeded)\n");
+      return 1;
+    }
+    h_u32 = to_f32(u0, cells);
+    CHECK(h3d_target_memcpy(d_ua, h_u32, cells * 4, o.device, -1));
+    CHECK(h3d_target_memcpy(d_ub, h_u32, cells * 4, o.device, -1));
+    if (o.bf16) {
+      uint16_t *kb = (uint16_t *)xmalloc(cells * 2);
+      CHECK(h3d_quantize_bf16(kap, kb, cells, &kscale));
+      /* the FP64 check runs on exactly the kappa the FP32 run sees */
+      for (size_t c = 0; c < cells; c++) {
+        uint32_t w = (uint32_t)kb[c] << 16;
+        float f;
+        memcpy(&f, &w, 4);
+        kap[c] = (double)f * kscale;
+      }
+      CHECK(h3d_target_alloc(&d_kap, cells * 2, o.device));
+      CHECK(h3d_target_memcpy(d_kap, kb, cells * 2, o.device, -1));
+      free(kb);
+    } else {
+      float *k32 = to_f32(kap, cells);
+      for (size_t c = 0; c < cells; c++) kap[c] = (double)k32[c];
+      CHECK(h3d_target_alloc(&d_kap, cells * 4, o.device));
+      CHECK(h3d_target_memcpy(d_kap, k32, cells * 4, o.device, -1));
+      free(k32);
+    }
+    if (o.prec == 3) {
+      CHECK(h3d_target_alloc(&d_ca, cells * 2, o.device));
+      CHECK(h3d_target_alloc(&d_cb, cells * 2, o.device));
+      uint16_t *z = (uint16_t *)calloc(cells, 2);
+      CHECK(h3d_target_memcpy(d_ca, z, cells * 2, o.device, -1));
+      CHECK(h3d_target_memcpy(d_cb, z, cells * 2, o.device, -1));
+      free(z);
+    }
+  }
+  const int mode = o.prec == 1 ? H3D_F32_PLAIN : o.prec == 2 ? H3D_F32_CARRY : H3D_F32_COMP16;
+  const float f32factor = (float)(factor * kscale);
+
+  h3d_tune tune = o.tune;
+  if (f32 && tune.kblock == 0) tune.kblock = 2;
+  if (o.autotune) {
+    h3d_tune t;
+    CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &g, mode,
+                       f32 ? o.tune.kblock : 0, stream, &t));
+    tune = t;
+    printf("Autotuned: rows=%d waves=%d zchunk=%d kblock=%d\n", tune.rows,
+           tune.waves, tune.zchunk, tune.kblock);
+  }
+
+  /* slab: face buffers for the in-HBM exchange */
+  double *d_face_lo = NULL, *d_face_hi = NULL;
+  if (o.slab_parts) {
+    CHECK(h3d_target_alloc((void **)&d_face_lo, (size_t)n * n * 8, o.device));
+    CHECK(h3d_target_alloc((void **)&d_face_hi, (size_t)n * n * 8, o.device));
+  }
+
+  printf("\n=== SIMULATION STARTS ===\n");
+  CHECK(h3d_device_synchronize(o.device));
+  double timer = -wtime();
+  int in_b = 0;
+  float dev_ms = 0.0f;
+  if (!f32 && !o.slab_parts && o.graph) {
+    CHECK(h3d_run_f64(d_ua, d_ub, d_rhs, d_kap, &g, factor, steps, &tune, 1,
+                      stream, &dev_ms, &in_b));
+  } else if (!f32) {
+    double *uo = d_ua, *un = d_ub;
+    for (int s = 0; s < steps; s++) {
+      if (o.slab_parts) {
+        /* middle-slab shape: bottom plane, top plane, then the interior
+         * (2-openmp.c:25-55); faces of u_new refreshed afterwards */
+        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 1, 1, &tune, stream));
+        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, nz - 2, nz - 2, &tune, stream));
+        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 2, nz - 3, &tune, stream));
+        CHECK(h3d_pack_face_f64(d_face_lo, un, &g, 1, stream));
+        CHECK(h3d_pack_face_f64(d_face_hi, un, &g, nz - 2, stream));
+        CHECK(h3d_unpack_face_f64(un, d_face_hi, &g, 0, stream));
+        CHECK(h3d_unpack_face_f64(un, d_face_lo, &g, nz - 1, stream));
+      } else {
+        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 1, nz - 2, &tune, stream));
+      }
+      double *t = uo; uo = un; un = t;
+    }
+    in_b = (uo == d_ub);
+  } else {
+    CHECK(h3d_run_f32(d_ua, d_ub, o.bf16 ? NULL : d_kap, o.bf16 ? d_kap : NULL,
+                      d_ca, d_cb, &g, f32factor, steps, mode, &tune, o.graph,
+                      stream, &dev_ms, &in_b));
+  }
+  CHECK(h3d_stream_synchronize(stream));
+  timer += wtime();
+
+  printf("\n%d timesteps complete\n", steps);
+  printf("Time = %f seconds\n", timer);
+  printf("=== SIMULATION ENDS ===\n");
+
+  const double lups = (double)(n - 2) * (n - 2) * (nz - 2) * steps;
+  double bpl = f32 ? (o.bf16 ? 10.0 : 12.0) : (rhs ? 32.0 : 24.0);
+  if (o.prec == 3) bpl += 4.0;
+  if (f32) bpl /= (tune.kblock > 0 ? tune.kblock : 1);
+  if (timer > 0)
+    printf("Performance: %.3f GLUP/s, %.1f GB/s algorithmic (%.2f B/LUP)\n",
+           lups / timer / 1e9, lups * bpl / timer / 1e9, bpl);
+
+  /* ---- diagnostics (FP64 reductions on the device) -------------------- */
+  void *d_res = in_b ? d_ub : d_ua;
+  double st[4];
+  if (f32) CHECK(h3d_stats_f32((const float *)d_res, &g, st, stream));
+  else CHECK(h3d_stats_f64((const double *)d_res, &g, st, stream));
+  const double h3 = h * h * h;
+  printf("checksum = %.17g  L2 = %.17g  max|u| = %.17g  energy = %.17g  nonfinite = %.0f\n",
+         st[0], sqrt(st[1] * h3), st[2], 0.5 * st[1] * h3, st[3]);
+
+  if (f32 && o.check) {
+    /* FP64 kernel on the same (rounded) inputs as the reference answer */
+    double *r0, *r1, *rk;
+    CHECK(h3d_target_alloc((void **)&r0, cells * 8, o.device));
+    CHECK(h3d_target_alloc((void **)&r1, cells * 8, o.device));
+    CHECK(h3d_target_alloc((void **)&rk, cells * 8, o.device));
+    for (size_t c = 0; c < cells; c++) u0[c] = (double)h_u32[c];
+    CHECK(h3d_target_memcpy(r0, u0, cells * 8, o.device, -1));
+    CHECK(h3d_target_memcpy(r1, u0, cells * 8, o.device, -1));
+    CHECK(h3d_target_memcpy(rk, kap, cells * 8, o.device, -1));
+    int rb = 0;
+    float rms = 0;
+    CHECK(h3d_run_f64(r0, r1, NULL, rk, &g, factor, steps, NULL, 0, stream, &rms, &rb));
+    double m[9];
+    CHECK(h3d_compare_f32(rb ? r1 : r0, (const float *)d_res, &g, 1e-6, 1e-5, m, stream));
+    printf("vs FP64: rel_l2 = %.3e  max_abs = %.3e  max_rel = %.3e  rms = %.3e  %s\n",
+           m[3], m[0], m[4], m[2], m[8] == 0 ? "PASS" : "FAIL");
+    h3d_target_free(r0, o.device);
+    h3d_target_free(r1, o.device);
+    h3d_target_free(rk, o.device);
+  }
+
+  if (should_write_txt()) {
+    double *out = (double *)xmalloc(cells * sizeof(double));
+    if (f32) {
+      float *o32 = (float *)xmalloc(cells * sizeof(float));
+      CHECK(h3d_target_memcpy(o32, d_res, cells * 4, -1, o.device));
+      for (size_t c = 0; c < cells; c++) out[c] = o32[c];
+      free(o32);
+    } else {
+      CHECK(h3d_target_memcpy(out, d_res, cells * 8, -1, o.device));
+    }
+    char name[64];
+    if (o.slab_parts)
+      snprintf(name, sizeof name, "u_h3d_slab%dof%d.txt", o.slab_index, o.slab_parts);
+    else
+      snprintf(name, sizeof name, "u_h3d_1gpu.txt");
+    CHECK(h3d_write_dump(name, out, &g));
+    free(out);
+  }
+
+  h3d_target_free(d_ua, o.device);
+  h3d_target_free(d_ub, o.device);
+  h3d_target_free(d_kap, o.device);
+  if (d_rhs) h3d_target_free(d_rhs, o.device);
+  if (d_ca) h3d_target_free(d_ca, o.device);
+  if (d_cb) h3d_target_free(d_cb, o.device);
+  if (d_face_lo) h3d_target_free(d_face_lo, o.device);
+  if (d_face_hi) h3d_target_free(d_face_hi, o.device);
+  h3d_stream_destroy(stream);
+  free(u0);
+  free(kap);
+  free(rhs);
+  free(h_u32);
+  printf("\n**** ONE GPU HIP (h3d, MI355X) FINISHED ****\n");
+  return 0;
+}

@@ -1,0 +1,166 @@
+"""FP64 kernel parity on the GPU: bit-exact against the reference's own
+serial program (golden fixtures) and against the CPU oracle."""
+import gzip
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INDEX = json.load(open(os.path.join(GOLDEN, "index.json")))
+
+
+@pytest.mark.parametrize("entry", INDEX, ids=lambda e: f"n{e['n']}_T{e['T']}")
+def test_reference_mirror_matches_golden(gpu, entry):
+    """h3d_heat_update (the reference's heat_update signature) in the
+    reference's own time loop reproduces u_serial.txt byte for byte."""
+    n = entry["n"]
+    h, dt, factor = gpu.params_reference(n)
+    steps = gpu.count_steps(float(entry["T"]), dt)
+    assert steps == entry["steps"]
+    u, rhs, k = gpu.init_reference(n)
+    du_old, du_new = gpu.DeviceArray.from_host(u), gpu.DeviceArray.from_host(u)
+    dr, dk = gpu.DeviceArray.from_host(rhs), gpu.DeviceArray.from_host(k)
+    for _ in range(steps):
+        gpu.call("h3d_heat_update", du_new.ptr, du_old.ptr, dr.ptr, dk.ptr, n, n, n, factor, 0)
+        du_old, du_new = du_new, du_old
+    gpu.call("h3d_device_synchronize", 0)
+    res = du_old.download()
+    with gzip.open(os.path.join(GOLDEN, entry["file"]), "rt") as f:
+        want = f.read().splitlines()
+    assert ["%.17g" % v for v in res] == want
+
+
+SHAPES = [
+    (5, 5, 5), (66, 66, 66), (67, 45, 33), (130, 9, 12), (3, 70, 8), (200, 3, 4),
+]
+TUNES = [None, (1, 1, 1), (2, 4, 7), (4, 2, 0), (8, 8, 3), (1, 16, 64)]
+
+
+def seeded(gpu, nx, ny, nz, seed=11, kind=0, pitch=0, rows_pad=0):
+    g = gpu.Grid(nx, ny, nz, nx + pitch, (nx + pitch) * (ny + rows_pad))
+    u, k = gpu.gen_seeded(g, 0, nx, ny, nz, kind, seed)
+    return g, u, k
+
+
+def oracle_steps(oracle, g, u, k, factor, steps, rhs=None):
+    return oracle.run_f64(u.copy(), u.copy(), rhs, k, g.nx, g.ny, g.nz, steps,
+                          sy=g.sy, sz=g.sz, factor=factor)
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=str)
+@pytest.mark.parametrize("tune", TUNES, ids=str)
+def test_step_matches_oracle_bitwise(gpu, oracle, shape, tune):
+    nx, ny, nz = shape
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1)
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    steps = 3
+    want = oracle_steps(oracle, g, u, k, factor, steps)
+    t = gpu.Tune(*tune, 1) if tune else None
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+    s = gpu.Stream()
+    for i in range(steps):
+        src, dst = (da, db) if i % 2 == 0 else (db, da)
+        gpu.call("h3d_step_f64", dst.ptr, src.ptr, None, dk.ptr, gpu.C.byref(g), factor,
+                 1, nz - 2, gpu.C.byref(t) if t else None, s.ptr)
+    s.sync()
+    got = (db if steps % 2 else da).download()
+    assert np.array_equal(got, want)
+
+
+def test_pitched_layout_and_rhs(gpu, oracle):
+    nx, ny, nz = 70, 21, 17
+    g, u, k = seeded(gpu, nx, ny, nz, pitch=10, rows_pad=3)
+    rhs = np.random.default_rng(4).uniform(-1e-4, 1e-4, g.elems)
+    factor = 0.02
+    want = oracle_steps(oracle, g, u, k, factor, 4, rhs=rhs)
+    da, db, dk, dr = (gpu.DeviceArray.from_host(x) for x in (u, u, k, rhs))
+    ms = gpu.C.c_float()
+    fb = gpu.C.c_int()
+    gpu.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr, dk.ptr, gpu.C.byref(g), factor, 4,
+             None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = (db if fb.value else da).download()
+    assert fb.value == 0
+    assert np.array_equal(got, want)
+    # padding between rows and planes is never written
+    pad = got.reshape(nz, ny + 3, nx + 10)
+    assert np.array_equal(pad[:, :, nx:], u.reshape(nz, ny + 3, nx + 10)[:, :, nx:])
+
+
+def test_interior_and_boundary_mirrors_equal_full_step(gpu, oracle):
+    n = 40
+    h, dt, factor = gpu.params_reference(n)
+    u, rhs, k = gpu.init_reference(n)
+    da, dr, dk = (gpu.DeviceArray.from_host(x) for x in (u, rhs, k))
+    full, split = gpu.DeviceArray.from_host(u), gpu.DeviceArray.from_host(u)
+    gpu.call("h3d_heat_update", full.ptr, da.ptr, dr.ptr, dk.ptr, n, n, n, factor, 0)
+    gpu.call("h3d_heat_update_boundary_bottom", split.ptr, da.ptr, dr.ptr, dk.ptr, n, n, n, factor, 0)
+    gpu.call("h3d_heat_update_boundary_top", split.ptr, da.ptr, dr.ptr, dk.ptr, n, n, n, factor, 0)
+    gpu.call("h3d_heat_update_interior", split.ptr, da.ptr, dr.ptr, dk.ptr, n, n, n, factor,
+             2, n - 3, 0)
+    gpu.call("h3d_device_synchronize", 0)
+    a, b = full.download(), split.download()
+    assert np.array_equal(a, b)
+    want = u.copy()
+    oracle.step_f64(want, u, rhs, k, n, n, n, factor, 1, n - 2)
+    assert np.array_equal(a, want)
+
+
+def test_graph_replay_equals_eager(gpu):
+    nx, ny, nz = 80, 50, 40
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1, seed=3)
+    factor = 0.01
+    res = []
+    s = gpu.Stream()
+    for graph in (0, 1):
+        da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+        ms, fb = gpu.C.c_float(), gpu.C.c_int()
+        gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), factor, 11,
+                 None, graph, s.ptr, gpu.C.byref(ms), gpu.C.byref(fb))
+        res.append((db if fb.value else da).download())
+        # replay again on the same buffers: graph replays after eager work
+        gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), factor, 6,
+                 None, graph, s.ptr, gpu.C.byref(ms), gpu.C.byref(fb))
+        res.append(da.download())
+    assert np.array_equal(res[0], res[2])
+    assert np.array_equal(res[1], res[3])
+
+
+def test_cfg1_128_cubed_100_steps(gpu, oracle):
+    """128^3 interior, seeded sine kappa, dt = 0.9 * bound, 100 steps: the
+    GPU must agree with the CPU oracle to <= 1e-12 max relative error; with
+    contraction off and the same association order it is bit-exact."""
+    m = 128
+    n = m + 2
+    g, u, k = seeded(gpu, n, n, n, seed=20260101)
+    h = 1.0 / (n - 1)
+    dt = 0.9 * 2 * h * h / gpu.lib().h3d_face_sum_max_host(gpu.ptr(k), gpu.C.byref(g))
+    factor = dt / h / h / 2.0
+    want = oracle.run_f64(u.copy(), u.copy(), None, k, n, n, n, 100, factor=factor, nthreads=8)
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), factor, 100,
+             None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = (db if fb.value else da).download()
+    m_ = oracle.metrics(want, got)
+    assert m_[4] <= 1e-12
+    assert np.array_equal(got, want)
+
+
+def test_argument_errors(gpu):
+    g = gpu.Grid.dense(10)
+    d = gpu.DeviceArray(g.elems * 8)
+    e = gpu.DeviceArray(g.elems * 8)
+    L = gpu.lib()
+    assert L.h3d_step_f64(d.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 1, 8, None, None) == -1
+    assert L.h3d_step_f64(e.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 0, 8, None, None) == -1
+    assert L.h3d_step_f64(e.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 1, 9, None, None) == -1
+    bad = gpu.Tune(3, 1, 1, 1)
+    assert L.h3d_step_f64(e.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 1, 8,
+                          gpu.C.byref(bad), None) == -1
+    assert L.h3d_heat_update(e.ptr, d.ptr, None, d.ptr, 10, 10, 10, 0.1, 7) == -3
+    # an empty range is a no-op, as the reference's loops are
+    assert L.h3d_step_f64(e.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 5, 4, None, None) == 0

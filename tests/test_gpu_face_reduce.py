@@ -1,0 +1,168 @@
+"""Face pack/unpack, the z-slab decomposition and the FP64 reductions on
+the GPU."""
+import math
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dims,pitch", [((7, 5, 4), 0), ((130, 66, 5), 0),
+                                        ((33, 17, 6), 5), ((1026, 3, 3), 0),
+                                        ((3, 1026, 3), 0)], ids=str)
+@pytest.mark.parametrize("f32", [False, True])
+def test_pack_unpack_bitwise(gpu, oracle, dims, pitch, f32):
+    nx, ny, nz = dims
+    g = gpu.Grid(nx, ny, nz, nx + pitch, (nx + pitch) * ny)
+    dt = np.float32 if f32 else np.float64
+    rng = np.random.default_rng(nx * ny)
+    u = rng.standard_normal(g.elems).astype(dt)
+    u.view(np.uint8)[::97] = 0xFF  # odd bit patterns (NaNs) travel unchanged
+    du = gpu.DeviceArray.from_host(u)
+    buf = gpu.DeviceArray(nx * ny * u.itemsize, dtype=dt)
+    sfx = "f32" if f32 else "f64"
+    s = gpu.Stream()
+    for k in (0, nz // 2, nz - 1):
+        gpu.call(f"h3d_pack_face_{sfx}", buf.ptr, du.ptr, gpu.C.byref(g), k, s.ptr)
+        s.sync()
+        got = buf.download()
+        want = u.reshape(nz, ny, nx + pitch)[k, :, :nx].ravel()
+        assert got.tobytes() == want.tobytes()
+    # unpack a different plane into k = 1 and compare whole arrays
+    face = rng.standard_normal(nx * ny).astype(dt)
+    buf.upload(face)
+    gpu.call(f"h3d_unpack_face_{sfx}", du.ptr, buf.ptr, gpu.C.byref(g), 1, s.ptr)
+    s.sync()
+    want = u.copy().reshape(nz, ny, nx + pitch)
+    want[1, :, :nx] = face.reshape(ny, nx)
+    assert du.download().tobytes() == want.ravel().tobytes()
+    L = gpu.lib()
+    assert L.h3d_pack_face_f64(buf.ptr, du.ptr, gpu.C.byref(g), nz, None) == -1
+
+
+@pytest.mark.parametrize("parts", [2, 4])
+def test_slabs_with_face_exchange_equal_global_run(gpu, oracle, parts):
+    """Split a reference-mode problem into z-slabs (the reference's
+    decomposition), step each slab with the boundary/interior kernel split
+    and exchange faces through pack -> unpack between slab buffers with the
+    reference's transfer map; the reconstructed field equals the global run
+    bit for bit (as the reference's multi-GPU variants equal its serial code,
+    HIP/verify-20370244.out:167-586)."""
+    n, steps = 38, 7
+    h, dt, factor = gpu.params_reference(n)
+    u, rhs, k = gpu.init_reference(n)
+    glob = oracle.run_f64(u.copy(), u.copy(), rhs, k, n, n, n, steps, factor=factor)
+    rc, nzd, gks = gpu.slab_decompose(n, parts)
+    assert rc == 0
+    L = gpu.lib()
+    slabs = []
+    for nz_l, gk in zip(nzd, gks):
+        a = [np.zeros(nz_l * n * n) for _ in range(3)]
+        assert L.h3d_init_partition(*(gpu.ptr(x) for x in a), nz_l, gk, 1, n, n, n, h, dt) == 0
+        su, sr, sk = a
+        slabs.append(dict(g=gpu.Grid.dense(n, n, nz_l),
+                          old=gpu.DeviceArray.from_host(su), new=gpu.DeviceArray.from_host(su),
+                          rhs=gpu.DeviceArray.from_host(sr), kap=gpu.DeviceArray.from_host(sk)))
+    face = gpu.DeviceArray(n * n * 8)
+    s = gpu.Stream()
+    for _ in range(steps):
+        for p, sl in enumerate(slabs):
+            g = sl["g"]
+            nz_l = g.nz
+            args = (sl["new"].ptr, sl["old"].ptr, sl["rhs"].ptr, sl["kap"].ptr, gpu.C.byref(g), factor)
+            gpu.call("h3d_step_f64", *args, 1, 1, None, s.ptr)
+            gpu.call("h3d_step_f64", *args, nz_l - 2, nz_l - 2, None, s.ptr)
+            if nz_l > 4:
+                gpu.call("h3d_step_f64", *args, 2, nz_l - 3, None, s.ptr)
+        # transfer map (4-openmp-stream-omp-p2p.c:609-616): plane k=nz-2 of
+        # slab p -> ghost k=0 of slab p+1; plane k=1 of slab p+1 -> ghost
+        # k=nz-1 of slab p.  Faces land in u_new, read as u_old next step.
+        for p in range(parts - 1):
+            lo, hi = slabs[p], slabs[p + 1]
+            gpu.call("h3d_pack_face_f64", face.ptr, lo["new"].ptr, gpu.C.byref(lo["g"]), lo["g"].nz - 2, s.ptr)
+            gpu.call("h3d_unpack_face_f64", hi["new"].ptr, face.ptr, gpu.C.byref(hi["g"]), 0, s.ptr)
+            gpu.call("h3d_pack_face_f64", face.ptr, hi["new"].ptr, gpu.C.byref(hi["g"]), 1, s.ptr)
+            gpu.call("h3d_unpack_face_f64", lo["new"].ptr, face.ptr, gpu.C.byref(lo["g"]), lo["g"].nz - 1, s.ptr)
+        for sl in slabs:
+            sl["old"], sl["new"] = sl["new"], sl["old"]
+    s.sync()
+    # reconstruction (4-openmp-stream-omp-p2p.c:776-820): owned planes only,
+    # plus the physical ghost planes of the first and last slab
+    out = np.zeros(n ** 3)
+    for p, (sl, gk) in enumerate(zip(slabs, gks)):
+        a = sl["old"].download().reshape(sl["g"].nz, n, n)
+        nxt = gks[p + 1] if p + 1 < parts else n
+        for kl in range(sl["g"].nz):
+            gk_g = kl - 1 + gk
+            if (p == 0 and gk_g == 0) or (gk <= gk_g < nxt):
+                out[gk_g * n * n:(gk_g + 1) * n * n] = a[kl].ravel()
+    assert np.array_equal(out, glob)
+
+
+def test_stats_matches_oracle(gpu, oracle):
+    n = 45
+    g = gpu.Grid(n, n - 3, n - 5, n + 3, (n + 3) * (n - 3))
+    u, k = gpu.gen_seeded(g, 0, n, n - 3, n - 5, 1, 8)
+    rng = np.random.default_rng(0)
+    u = u * 10 ** rng.uniform(-3, 3, u.size)
+    du = gpu.DeviceArray.from_host(u)
+    got = gpu.stats(du, g)
+    want = oracle.stats(u, g.nx, g.ny, g.nz, g.sy, g.sz)
+    assert got[0] == pytest.approx(want[0], rel=1e-14)
+    assert got[1] == pytest.approx(want[1], rel=1e-14)
+    assert got[2] == want[2] and got[3] == 0
+    again = gpu.stats(du, g)
+    assert np.array_equal(got, again)  # deterministic
+    u32 = u.astype(np.float32)
+    got32 = gpu.stats(gpu.DeviceArray.from_host(u32), g, f32=True)
+    want32 = oracle.stats(u32.astype(np.float64), g.nx, g.ny, g.nz, g.sy, g.sz)
+    assert got32[0] == pytest.approx(want32[0], rel=1e-14)
+    u[17] = np.nan
+    u[40] = np.inf
+    assert gpu.stats(gpu.DeviceArray.from_host(u), g)[3] == 2
+
+
+def test_compare_matches_oracle_metrics(gpu, oracle):
+    nx, ny, nz = 50, 40, 30
+    g = gpu.Grid.dense(nx, ny, nz)
+    rng = np.random.default_rng(3)
+    ref = rng.standard_normal(g.elems)
+    ref[::50] = 0.0
+    cand = ref + rng.standard_normal(g.elems) * 1e-6 * np.abs(ref)
+    cand[123] += 1e-3
+    cand[777] = np.inf
+    dr, dc = gpu.DeviceArray.from_host(ref), gpu.DeviceArray.from_host(cand)
+    got = gpu.compare(dr, dc, g, 1e-6, 1e-5)
+    want = oracle.metrics(ref, cand, 1e-6, 1e-5)
+    assert got[5] == want[5] == 777
+    for i in (6, 7, 8):
+        assert got[i] == want[i]
+    fin = cand.copy()
+    fin[777] = ref[777]
+    dc.upload(fin)
+    got = gpu.compare(dr, dc, g, 1e-6, 1e-5)
+    want = oracle.metrics(ref, fin, 1e-6, 1e-5)
+    assert got[0] == want[0] and got[5] == want[5] == 123
+    for i in (1, 2, 3, 4):
+        assert got[i] == pytest.approx(want[i], rel=1e-12)
+    assert got[8] == want[8]
+    c32 = fin.astype(np.float32)
+    got = gpu.compare(dr, gpu.DeviceArray.from_host(c32), g, 1e-6, 1e-5, f32=True)
+    want = oracle.metrics(ref, c32.astype(np.float64), 1e-6, 1e-5)
+    assert got[3] == pytest.approx(want[3], rel=1e-12) and got[8] == want[8]
+
+
+def test_face_sum_max_device_matches_host(gpu, oracle):
+    n = 61
+    g = gpu.Grid.dense(n, n - 7, n - 11)
+    _, k = gpu.gen_seeded(g, 0, g.nx, g.ny, g.nz, 1, 21)
+    dk = gpu.DeviceArray.from_host(k)
+    out = np.zeros(1)
+    gpu.call("h3d_face_sum_max_f64", dk.ptr, gpu.C.byref(g), gpu.ptr(out), None)
+    assert out[0] == oracle.face_sum_max(k, g.nx, g.ny, g.nz)
+
+
+def test_stream_probe_runs(gpu):
+    c, t = gpu.stream_probe(1 << 28, reps=2)
+    assert c > 100.0 and t > 100.0 and math.isfinite(c)

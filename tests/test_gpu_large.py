@@ -1,0 +1,87 @@
+"""Full-size checks at the benchmark shape (512^3 interior, seeded sine
+kappa, dt = 0.9 x bound): whole-field bit-exactness against the CPU oracle
+over the first steps, then size-independent properties over 1000 steps
+(discrete maximum principle, non-increasing energy, deterministic
+checksum) and the FP32-compensated accuracy target against FP64."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+M = 512
+N = M + 2
+
+
+@pytest.fixture(scope="module")
+def big(gpu):
+    g = gpu.Grid.dense(N)
+    u, k = gpu.gen_seeded(g, 0, N, N, N, 0, 20260101)
+    h = 1.0 / (N - 1)
+    fmax = gpu.lib().h3d_face_sum_max_host(gpu.ptr(k), gpu.C.byref(g))
+    dt = 0.9 * 2 * h * h / fmax
+    return dict(g=g, u=u, k=k, factor=dt / h / h / 2.0)
+
+
+def run64(gpu, P, u0, k, steps):
+    g = P["g"]
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u0, u0, k))
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), P["factor"],
+             steps, None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    return (db, da) if fb.value else (da, db)
+
+
+def test_full_field_bitwise_vs_oracle(gpu, oracle, big):
+    steps = 12
+    res, _ = run64(gpu, big, big["u"], big["k"], steps)
+    got = res.download()
+    want = oracle.run_f64(big["u"].copy(), big["u"].copy(), None, big["k"], N, N, N,
+                          steps, factor=big["factor"], nthreads=16)
+    assert np.array_equal(got, want)
+    sub = (slice(1, N - 1, 8),) * 3  # the 64^3 strided subsample
+    m = oracle.metrics(want.reshape(N, N, N)[sub].ravel(), got.reshape(N, N, N)[sub].ravel())
+    assert m[4] <= 1e-12
+
+
+def test_1000_steps_invariants(gpu, big):
+    g = big["g"]
+    u0 = big["u"]
+    s0 = gpu.stats(gpu.DeviceArray.from_host(u0), g)
+    prev = s0
+    cur = u0
+    for chunk in range(4):
+        res, _ = run64(gpu, big, cur, big["k"], 250)
+        st = gpu.stats(res, g)
+        assert st[3] == 0
+        assert st[2] <= prev[2] * (1 + 1e-15)  # max principle (u >= 0, no source)
+        assert st[1] <= prev[1]               # energy does not grow
+        cur = res.download()
+        prev = st
+    res2, _ = run64(gpu, big, u0, big["k"], 1000)
+    assert np.array_equal(gpu.stats(res2, g), prev)  # checksum is deterministic
+
+
+def test_fp32_compensated_target_accuracy(gpu, big):
+    """The north-star accuracy bar: FP32 storage, compensated, temporally
+    blocked, 1000 steps at 512^3: rel-L2 <= 1e-5 against the FP64 kernel on
+    the same (FP32-rounded) inputs."""
+    g = big["g"]
+    u32 = big["u"].astype(np.float32)
+    k32 = big["k"].astype(np.float32)
+    ref, _ = run64(gpu, big, u32.astype(np.float64), k32.astype(np.float64), 1000)
+    da, db = gpu.DeviceArray.from_host(u32), gpu.DeviceArray.from_host(u32)
+    dk = gpu.DeviceArray.from_host(k32)
+    f32f = gpu.C.c_float(np.float32(big["factor"]))
+    results = {}
+    for kb in (2, 4):
+        da.upload(u32)
+        db.upload(u32)
+        ms, fb = gpu.C.c_float(), gpu.C.c_int()
+        t = gpu.Tune(0, 0, 0, kb)
+        gpu.call("h3d_run_f32", da.ptr, db.ptr, dk.ptr, None, None, None, gpu.C.byref(g),
+                 f32f, 1000, 1, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+        m = gpu.compare(ref, db if fb.value else da, g, f32=True)
+        results[kb] = (m[3], m[0])
+    print("rel-L2, max-abs vs FP64:", results)
+    for kb, (rel_l2, max_abs) in results.items():
+        assert rel_l2 <= 1e-5, kb

@@ -1,0 +1,308 @@
+"""Pins the CPU oracle against evidence other than itself.
+
+- golden fixtures written by the reference's own serial program
+  (OpenMP_Single_Source/heat3D_variable_coeff.c compiled from source, see
+  tests/golden/make_golden.py), compared byte for byte;
+- values recorded in the reference's logs (HIP/verify-20370244.out:115-150,
+  HIP/summary.csv steps column);
+- closed forms (the discrete eigenmode of the constant-kappa operator);
+- brute-force numpy restatements, and mutants of them that must fail the
+  golden fixtures (a dropped term, a flipped sign, a transposed index).
+"""
+import gzip
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+INDEX = json.load(open(os.path.join(GOLDEN, "index.json")))
+
+
+def golden_lines(entry):
+    with gzip.open(os.path.join(GOLDEN, entry["file"]), "rt") as f:
+        return f.read().splitlines()
+
+
+def dump_lines(u):
+    return ["%.17g" % v for v in u]
+
+
+@pytest.mark.parametrize("entry", INDEX, ids=lambda e: f"n{e['n']}_T{e['T']}")
+def test_oracle_matches_reference_binary_bitwise(oracle, entry):
+    n = entry["n"]
+    h, dt, factor = oracle.params_reference(n)
+    steps = oracle.count_steps(float(entry["T"]), dt)
+    assert steps == entry["steps"]
+    u, un, rhs, k = oracle.init_reference(n)
+    res = oracle.run_f64(u, un, rhs, k, n, n, n, steps, factor=factor)
+    assert dump_lines(res) == golden_lines(entry)
+
+
+@pytest.mark.parametrize("entry", INDEX, ids=lambda e: f"n{e['n']}_T{e['T']}")
+def test_oracle_openmp_equals_serial(oracle, entry):
+    n = entry["n"]
+    h, dt, factor = oracle.params_reference(n)
+    steps = entry["steps"]
+    u, un, rhs, k = oracle.init_reference(n)
+    a = oracle.run_f64(u.copy(), un.copy(), rhs, k, n, n, n, steps, factor=factor)
+    b = oracle.run_f64(u.copy(), un.copy(), rhs, k, n, n, n, steps, factor=factor,
+                       nthreads=4)
+    assert np.array_equal(a, b)
+
+
+def test_dump_format_roundtrip(oracle, tmp_path):
+    entry = INDEX[-1]
+    n = entry["n"]
+    h, dt, factor = oracle.params_reference(n)
+    u, un, rhs, k = oracle.init_reference(n)
+    res = oracle.run_f64(u, un, rhs, k, n, n, n, entry["steps"], factor=factor)
+    path = str(tmp_path / "u.txt")
+    assert oracle.lib().orc_write_dump(path.encode(), oracle.p(res), n) == 0
+    text = open(path).read().splitlines()
+    assert text == golden_lines(entry)
+    assert np.array_equal(np.array([float(x) for x in text]), res)  # %.17g round-trips
+
+
+def test_step_counts_recorded_by_reference(oracle):
+    # T = STEPS*dt printed %.7g (submit.sh:61-62); the loop test
+    # `while (t<T)` then runs STEPS or STEPS+1 times.  Recorded counts:
+    # HIP/verify-20370244.out:116 (N=256, 100 -> 101) and the steps column of
+    # AMD-MI250X/Source-Results/HIP/summary.csv:2-8 (STEPS=500).
+    recorded = {(256, 100): 101, (512, 500): 501, (640, 500): 500, (768, 500): 501,
+                (896, 500): 501, (1024, 500): 501, (1152, 500): 500, (1280, 500): 500}
+    for (n, s), want in recorded.items():
+        h, dt, f = oracle.params_reference(n)
+        T = float("%.7g" % (s * dt))
+        assert oracle.count_steps(T, dt) == want, (n, s)
+
+
+def test_params_match_recorded_banner(oracle):
+    # HIP/verify-20370244.out:141-150 (N=256)
+    h, dt, factor = oracle.params_reference(256)
+    assert "%.6e" % h == "3.921569e-03"
+    assert "%.6e" % dt == "1.349009e-06"
+    assert "%.6e" % factor == "4.385965e-02"
+    assert "%.6f" % (dt / (h * h)) == "0.087719"
+    # factor is dt/h/h/2 evaluated left to right, not dt/(2h^2)
+    assert factor == dt / h / h / 2.0
+
+
+def numpy_step(u, k, rhs, factor, mutate=None):
+    """Brute-force restatement of heat3D_variable_coeff.c:174-181."""
+    c = u[1:-1, 1:-1, 1:-1]
+    kc = k[1:-1, 1:-1, 1:-1]
+
+    def nb(a, dz, dy, dx):
+        n0, n1, n2 = a.shape
+        return a[1 + dz:n0 - 1 + dz, 1 + dy:n1 - 1 + dy, 1 + dx:n2 - 1 + dx]
+
+    A = (nb(k, 0, 0, 1) + kc) * (nb(u, 0, 0, 1) - c)
+    B = (kc + nb(k, 0, 0, -1)) * (c - nb(u, 0, 0, -1))
+    Cc = (nb(k, 0, 1, 0) + kc) * (nb(u, 0, 1, 0) - c)
+    D = (kc + nb(k, 0, -1, 0)) * (c - nb(u, 0, -1, 0))
+    E = (nb(k, 1, 0, 0) + kc) * (nb(u, 1, 0, 0) - c)
+    F = (kc + nb(k, -1, 0, 0)) * (c - nb(u, -1, 0, 0))
+    if mutate == "drop_E":
+        E = 0.0 * E
+    elif mutate == "sign_D":
+        D = -D
+    elif mutate == "transpose_C":  # j+1 neighbour taken along x
+        Cc = (nb(k, 0, 0, 1) + kc) * (nb(u, 0, 1, 0) - c)
+    S = A - B + Cc - D + E - F
+    out = u.copy()
+    out[1:-1, 1:-1, 1:-1] = c + nb(rhs, 0, 0, 0) + factor * S
+    return out
+
+
+@pytest.mark.parametrize("mutate", [None, "drop_E", "sign_D", "transpose_C"])
+def test_golden_pins_every_term(oracle, mutate):
+    entry = next(e for e in INDEX if e["n"] == 9)
+    n, steps = entry["n"], entry["steps"]
+    h, dt, factor = oracle.params_reference(n)
+    u, un, rhs, k = oracle.init_reference(n)
+    u, rhs, k = (a.reshape(n, n, n) for a in (u, rhs, k))
+    for _ in range(steps):
+        u = numpy_step(u, k, rhs, factor, mutate)
+    same = dump_lines(u.ravel()) == golden_lines(entry)
+    assert same == (mutate is None)
+
+
+def test_constant_kappa_eigenmode_decay(oracle):
+    # kappa = c, rhs = 0, u0 = sin(pi x) sin(pi y) sin(pi z): each step
+    # multiplies u by g = 1 - 12*factor*c*(1 - cos(pi h)).
+    n, c, steps = 21, 0.7, 40
+    h, dt, factor = oracle.params_reference(n)
+    x = np.arange(n) * h
+    s = np.sin(np.pi * x)
+    u0 = (s[:, None, None] * s[None, :, None] * s[None, None, :]).ravel()
+    u0 = u0.reshape(n, n, n)
+    u0[0], u0[-1], u0[:, 0], u0[:, -1], u0[:, :, 0], u0[:, :, -1] = 0, 0, 0, 0, 0, 0
+    u0 = u0.ravel().copy()
+    k = np.full(n ** 3, c)
+    res = oracle.run_f64(u0.copy(), u0.copy(), None, k, n, n, n, steps, factor=factor)
+    g = 1.0 - 12.0 * factor * c * (1.0 - math.cos(math.pi * h))
+    assert np.max(np.abs(res - g ** steps * u0)) < 1e-13
+
+
+def test_interior_boundary_split_equals_full(oracle):
+    n = 11
+    h, dt, factor = oracle.params_reference(n)
+    u, un, rhs, k = oracle.init_reference(n)
+    full = un.copy()
+    oracle.step_f64(full, u, rhs, k, n, n, n, factor, 1, n - 2)
+    split = un.copy()
+    oracle.step_f64(split, u, rhs, k, n, n, n, factor, 1, 1)
+    oracle.step_f64(split, u, rhs, k, n, n, n, factor, n - 2, n - 2)
+    oracle.step_f64(split, u, rhs, k, n, n, n, factor, 2, n - 3)
+    assert np.array_equal(full, split)
+
+
+def test_slab_decompose_closed_forms(oracle):
+    # 4-openmp-stream-omp-p2p.c:391-415: remainder to the lower devices
+    assert oracle.slab_decompose(1026, 4) == (0, [258] * 4, [1, 257, 513, 769])
+    assert oracle.slab_decompose(1024, 4) == (0, [258, 258, 257, 257], [1, 257, 513, 768])
+    # 2-openmp.c:294-316: remainder to the higher device
+    assert oracle.slab_decompose(1025, 2) == (0, [513, 514], [1, 512])
+    assert oracle.slab_decompose(9, 4)[0] == -1  # slabs of 3 planes are refused
+    assert oracle.slab_decompose(10, 4)[0] == 0
+
+
+def test_slab_partition_init_matches_global(oracle):
+    n, parts = 14, 4
+    h, dt, f = oracle.params_reference(n)
+    u, _, rhs, k = oracle.init_reference(n)
+    rc, nzd, gks = oracle.slab_decompose(n, parts)
+    for nz_l, gk in zip(nzd, gks):
+        su, sr, sk = (np.empty(nz_l * n * n) for _ in range(3))
+        oracle.lib().orc_init_partition(oracle.p(su), oracle.p(sr), oracle.p(sk),
+                                        nz_l, gk, 1, n, n, n, h, dt)
+        lo = (gk - 1) * n * n
+        assert np.array_equal(sr, rhs[lo:lo + nz_l * n * n])
+        assert np.array_equal(sk, k[lo:lo + nz_l * n * n])
+        assert np.array_equal(su, u[lo:lo + nz_l * n * n])
+
+
+def test_metrics_hand_vectors(oracle):
+    ref = np.array([1.0, -2.0, 0.0, 4.0, 1e-13])
+    cand = np.array([1.0, -2.5, 1e-7, 4.0, 2e-13])
+    m = oracle.metrics(ref, cand, atol=1e-6, rtol=1e-5)
+    d = cand - ref
+    assert m[0] == 0.5 and m[5] == 1
+    assert m[1] == pytest.approx(np.abs(d).mean(), rel=1e-15)
+    assert m[2] == pytest.approx(math.sqrt((d ** 2).mean()), rel=1e-15)
+    assert m[3] == pytest.approx(np.linalg.norm(d) / np.linalg.norm(ref), rel=1e-15)
+    assert m[4] == pytest.approx(0.25, rel=1e-15)  # |ref| > 1e-12 only
+    assert m[6] == 0 and m[7] == 0
+    assert m[8] == 1  # only the -2 -> -2.5 cell fails atol + rtol*|ref|
+    bad = cand.copy()
+    bad[2] = np.nan
+    bad[3] = np.inf
+    m = oracle.metrics(ref, bad)
+    assert m[6] == 1 and m[7] == 1 and m[8] == 3
+    assert math.isnan(m[0]) and m[5] == 2  # numpy: argmax returns the first NaN
+
+
+def test_stats_against_fsum(oracle):
+    rng = np.random.default_rng(5)
+    u = rng.standard_normal(7 * 6 * 5) * 10 ** rng.uniform(-8, 8, 7 * 6 * 5)
+    s = oracle.stats(u, 7, 6, 5)
+    assert s[0] == pytest.approx(math.fsum(u), rel=1e-15, abs=1e-300)
+    assert s[1] == pytest.approx(math.fsum(u * u), rel=1e-15)
+    assert s[2] == np.abs(u).max()
+    u[3] = np.nan
+    assert oracle.stats(u, 7, 6, 5)[3] == 1
+
+
+def test_face_sum_max_brute_force(oracle):
+    rng = np.random.default_rng(1)
+    nz, ny, nx = 5, 6, 7
+    k = rng.uniform(0.5, 2.0, nz * ny * nx)
+    K = k.reshape(nz, ny, nx)
+    best = 0.0
+    for z in range(1, nz - 1):
+        for y in range(1, ny - 1):
+            for x in range(1, nx - 1):
+                c = K[z, y, x]
+                s = sum(K[z + dz, y + dy, x + dx] + c for dz, dy, dx in
+                        [(0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (1, 0, 0), (-1, 0, 0)])
+                best = max(best, s)
+    assert oracle.face_sum_max(k, nx, ny, nz) == pytest.approx(best, rel=1e-15)
+    # constant kappa: 12 c, so dt <= 2h^2/(12c) = h^2/(6c)
+    assert oracle.face_sum_max(np.full(nz * ny * nx, 0.75), nx, ny, nz) == 9.0
+
+
+# ---------------------------------------------------------------------------
+# FP32 paths (no reference counterpart).
+
+def f32_problem(nx, ny, nz, seed=3):
+    rng = np.random.default_rng(seed)
+    u = rng.uniform(0, 1, (nz, ny, nx)).astype(np.float32)
+    u[0], u[-1], u[:, 0], u[:, -1], u[:, :, 0], u[:, :, -1] = 0, 0, 0, 0, 0, 0
+    k = rng.uniform(0.5, 1.5, (nz, ny, nx)).astype(np.float32)
+    return u.ravel(), k.ravel()
+
+
+def numpy_f32_plain(u, k, factor, steps, shape):
+    u = u.reshape(shape).copy()
+    k = k.reshape(shape)
+    f = np.float32(factor)
+    for _ in range(steps):
+        u = numpy_step(u, k, np.zeros_like(u), f).astype(np.float32)
+    return u.ravel()
+
+
+def test_f32_plain_matches_numpy(oracle):
+    nx, ny, nz = 9, 8, 7
+    u, k = f32_problem(nx, ny, nz)
+    factor = 0.05
+    got, _ = oracle.run_f32(u, k, None, nx, ny, nz, factor, 6, 2, 0)
+    with np.errstate(all="ignore"):
+        # numpy float32 arithmetic rounds each operation like the C code
+        want = numpy_f32_plain(u, k, factor, 6, (nz, ny, nx))
+    assert np.array_equal(got, want)
+
+
+def test_f32_compensated_is_closer_to_f64(oracle):
+    nx, ny, nz = 12, 11, 10
+    u, k = f32_problem(nx, ny, nz, seed=9)
+    u64 = u.astype(np.float64)
+    k64 = k.astype(np.float64)
+    factor = np.float32(0.04)
+    steps = 400
+    ref = oracle.run_f64(u64.copy(), u64.copy(), None, k64, nx, ny, nz, steps,
+                         factor=float(factor))
+    err = {}
+    for mode, kb in [(0, 4), (1, 4), (2, 4), (2, 1)]:
+        got, _ = oracle.run_f32(u, k, None, nx, ny, nz, float(factor), steps, kb, mode)
+        err[(mode, kb)] = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    assert err[(1, 4)] < err[(0, 4)]
+    assert err[(2, 4)] < err[(1, 4)]
+    assert err[(2, 1)] < 1e-6
+
+
+def test_comp16_encode_decode(oracle):
+    L = oracle.lib()
+    for u in [1.0, 3.7, 1e-5, -2.5e3, 0.0]:
+        u32 = np.float32(u)
+        ulp = np.spacing(np.abs(u32)) if u != 0 else np.float32(2.0 ** -149)
+        for frac in [0.0, 0.5, -0.5, 0.123, -0.377]:
+            e = np.float32(frac * ulp)
+            back = L.orc_comp_decode(L.orc_comp_encode(e, u32), u32)
+            assert abs(back - e) <= abs(float(ulp)) * 2.0 ** -11
+    # exact for values representable in FP16 relative to the ulp
+    assert L.orc_comp_decode(L.orc_comp_encode(np.float32(2.0 ** -24), 1.0), 1.0) == 2.0 ** -24
+
+
+def test_bf16_conversion_matches_numpy(oracle):
+    L = oracle.lib()
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-4, 4, 2000).astype(np.float32)
+    w = x.view(np.uint32).astype(np.uint64)
+    want = ((w + 0x7FFF + ((w >> 16) & 1)) >> 16).astype(np.uint16)
+    got = np.array([L.orc_f32_to_bf16(float(v)) for v in x], dtype=np.uint16)
+    assert np.array_equal(got, want)
+    back = np.array([L.orc_bf16_to_f32(int(b)) for b in got], dtype=np.float32)
+    assert np.all(np.abs(back - x) <= np.abs(x) * 2.0 ** -8)
