@@ -56,6 +56,9 @@ def parse():
                     help="FP32 time steps per pass (0: autotune)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
+    ap.add_argument("--tune64", default="", help="rows,waves,zchunk: skip FP64 autotuning")
+    ap.add_argument("--tune32", default="",
+                    help="kblock,rows,waves,zchunk: skip FP32 autotuning")
     return ap.parse_args()
 
 
@@ -147,9 +150,13 @@ def main():
 
     stream = h3d.Stream()
     da, db, dk = (h3d.DeviceArray.from_host(x) for x in (u0, u0, kap))
-    tune = h3d.Tune()
-    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0,
-             stream.ptr, h3d.C.byref(tune))
+    if a.tune64:
+        r_, w_, z_ = (int(x) for x in a.tune64.split(","))
+        tune = h3d.Tune(r_, w_, z_, 1)
+    else:
+        tune = h3d.Tune()
+        h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0,
+                 stream.ptr, h3d.C.byref(tune))
     log(f"FP64 tune: rows={tune.rows} waves={tune.waves} zchunk={tune.zchunk}")
 
     ms, fb = h3d.C.c_float(), h3d.C.c_int()
@@ -194,7 +201,7 @@ def main():
                      "kernel_ms_per_launch": round(1e3 * per_launch_s, 5)},
         "stream": {"copy_gbs": round(copy_gbs, 1), "triad_gbs": round(triad_gbs, 1),
                    "array_bytes": 4 << 30},
-        "checksum": {"sum": st[0], "sumsq": st[1], "max": st[2]},
+        "checksum": {"sum": float(st[0]), "sumsq": float(st[1]), "max": float(st[2])},
     }
 
     # ---- FP32 compensated, temporally blocked ----
@@ -210,9 +217,13 @@ def main():
         ref = db if fb.value else da
         fa, fbuf, fk = (h3d.DeviceArray.from_host(x) for x in (u32, u32, k32))
         f32f = h3d.C.c_float(np.float32(factor))
-        t32 = h3d.Tune()
-        h3d.call("h3d_autotune", 1, fa.ptr, fbuf.ptr, fk.ptr, 0, h3d.C.byref(g), 1,
-                 a.kblock, stream.ptr, h3d.C.byref(t32))
+        if a.tune32:
+            k_, r_, w_, z_ = (int(x) for x in a.tune32.split(","))
+            t32 = h3d.Tune(r_, w_, z_, k_)
+        else:
+            t32 = h3d.Tune()
+            h3d.call("h3d_autotune", 1, fa.ptr, fbuf.ptr, fk.ptr, 0, h3d.C.byref(g), 1,
+                     a.kblock, stream.ptr, h3d.C.byref(t32))
         log(f"FP32 tune: kblock={t32.kblock} rows={t32.rows} waves={t32.waves} "
             f"zchunk={t32.zchunk}")
         h3d.call("h3d_run_f32", fa.ptr, fbuf.ptr, fk.ptr, None, None, None, h3d.C.byref(g),
@@ -241,10 +252,11 @@ def main():
             "achieved_gbs": round(gbs32, 1), "frac_of_stream": round(gbs32 / bw, 4),
             "algorithmic_bytes_per_lup": round(12.0 / t32.kblock, 3),
             "speedup_vs_fp64": round(glups32 / glups, 3),
-            "rel_l2_vs_fp64": m32[3], "max_abs_vs_fp64": m32[0], "max_rel_vs_fp64": m32[4],
-            "targets": {"frac_of_stream>=0.8": gbs32 / bw >= 0.8,
-                        "speedup>=1.8": glups32 / glups >= 1.8,
-                        "rel_l2<=1e-5": m32[3] <= 1e-5},
+            "rel_l2_vs_fp64": float(m32[3]), "max_abs_vs_fp64": float(m32[0]),
+            "max_rel_vs_fp64": float(m32[4]),
+            "targets": {"frac_of_stream>=0.8": bool(gbs32 / bw >= 0.8),
+                        "speedup>=1.8": bool(glups32 / glups >= 1.8),
+                        "rel_l2<=1e-5": bool(m32[3] <= 1e-5)},
         }
 
     # ---- CPU baseline ----

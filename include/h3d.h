@@ -48,7 +48,7 @@ typedef struct h3d_tune {
   int rows;   /* y rows per lane held in registers: 1, 2, 4 or 8     */
   int waves;  /* wavefronts per workgroup, stacked in y: 1..16        */
   int zchunk; /* planes marched by one workgroup (0 = whole range)    */
-  int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..8    */
+  int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..6    */
 } h3d_tune;
 
 /* ---------------------------------------------------------------------- */
@@ -146,12 +146,12 @@ int h3d_face_sum_max_f64(const double *kappa, const h3d_grid *g, double *out,
 #define H3D_F32_COMP16 2 /* compensated; residual also kept between passes
                            in an FP16 field scaled by the cell's ulp      */
 
-/* One HBM pass of `nsteps` (1..8) fused time steps over the interior of
+/* One HBM pass of `nsteps` (1..6) fused time steps over the interior of
  * a grid stored in FP32.  kappa is exactly one of kappa_f32 / kappa_bf16
  * (BF16 bits of kappa/scale; pass factor*scale as `factor`).  comp_in /
  * comp_out (FP16 bits) are used in mode H3D_F32_COMP16 only.  Cell update:
- * S in the FP64 kernel's order in FP32, d = factor*S, then TwoSum of u and
- * d + e.  u_out's ring must hold the Dirichlet values. */
+ * S in the FP64 kernel's order in FP32, d = factor*S, then Kahan's
+ * compensated sum: y = d + e, u' = u + y, e' = y - (u' - u).  u_out's ring must hold the Dirichlet values. */
 int h3d_pass_f32(float *u_out, const float *u_in, const float *kappa_f32,
                  const uint16_t *kappa_bf16, uint16_t *comp_out,
                  const uint16_t *comp_in, const h3d_grid *g, float factor,

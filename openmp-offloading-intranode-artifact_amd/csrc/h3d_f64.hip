@@ -27,7 +27,7 @@
 // Every HBM byte of u_old and kappa is read once per step from the
 // memory side (halo re-reads hit the caches), and u_new written once:
 // 24 B per lattice update (32 B with rhs).
-#include "h3d_internal.h"
+#include "h3d_tiles.h"
 
 namespace {
 
@@ -37,136 +37,13 @@ h3d_march_f64(double *__restrict__ un, const double *__restrict__ uo,
               const double *__restrict__ rhs, const double *__restrict__ kap,
               int nx, int ny, int sy, long long sz, double factor,
               int k_start, int k_end, int zchunk, int tiles_x, int tiles_y) {
-#pragma clang fp contract(off)
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int waves = blockDim.x >> 6;
   unsigned t = h3d_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = (int)(t % (unsigned)tiles_x);
   t /= (unsigned)tiles_x;
   const int ty = (int)(t % (unsigned)tiles_y);
   const int tz = (int)(t / (unsigned)tiles_y);
-
-  const int i = 1 + tx * 64 + lane;
-  const int j0 = 1 + (ty * waves + wave) * RY;
-  if (j0 > ny - 2) return;  // wave-uniform; the kernel has no barriers
-  const int kb = k_start + tz * zchunk;
-  if (kb > k_end) return;
-  const int ke = min(k_end, kb + zchunk - 1);
-
-  const bool lane_ld = i <= nx - 1;   // this lane's column exists
-  const bool lane_cmp = i <= nx - 2;  // and is interior
-  // x halo: lane 0 holds the point left of the segment, lane 63 the point
-  // right of it (needed only when lane 63 itself is interior).
-  const bool xh_ld = (lane == 0) || (lane == 63 && lane_cmp);
-  const int xh_di = (lane == 0) ? -1 : 1;
-
-  int off[RY];
-  bool row_ld[RY], row_cmp[RY];
-#pragma unroll
-  for (int r = 0; r < RY; ++r) {
-    const int j = j0 + r;
-    off[r] = j * sy + i;
-    row_ld[r] = j <= ny - 1;
-    row_cmp[r] = j <= ny - 2;
-  }
-  const int lo_off = (j0 - 1) * sy + i;
-  const int hi_off = (j0 + RY) * sy + i;
-  const bool hi_ld = j0 + RY <= ny - 1;
-
-  auto ld = [&](const double *p, int o, bool pred) -> double {
-    return pred ? p[o] : 0.0;
-  };
-
-  double um[RY], uc[RY], up[RY], km[RY], kc[RY], kq[RY];
-  double uxh[RY], kxh[RY], ulo, uhi, klo, khi;
-  double rc[RY];
-
-  {
-    const double *u0 = uo + (long long)(kb - 1) * sz;
-    const double *k0 = kap + (long long)(kb - 1) * sz;
-    const double *u1 = u0 + sz, *k1 = k0 + sz;
-    const double *u2 = u1 + sz, *k2 = k1 + sz;
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const bool p = lane_ld && row_ld[r];
-      um[r] = ld(u0, off[r], p);
-      km[r] = ld(k0, off[r], p);
-      uc[r] = ld(u1, off[r], p);
-      kc[r] = ld(k1, off[r], p);
-      up[r] = ld(u2, off[r], p);
-      kq[r] = ld(k2, off[r], p);
-      uxh[r] = ld(u1, off[r] + xh_di, xh_ld && row_cmp[r]);
-      kxh[r] = ld(k1, off[r] + xh_di, xh_ld && row_cmp[r]);
-      if (RHS) rc[r] = ld(rhs + (long long)kb * sz, off[r], p);
-    }
-    ulo = ld(u1, lo_off, lane_ld);
-    klo = ld(k1, lo_off, lane_ld);
-    uhi = ld(u1, hi_off, lane_ld && hi_ld);
-    khi = ld(k1, hi_off, lane_ld && hi_ld);
-  }
-
-  for (int k = kb; k <= ke; ++k) {
-    // Prefetch: halos of plane k+1 and rows of plane k+2.
-    double nu[RY], nk[RY], nuxh[RY], nkxh[RY], nulo, nuhi, nklo, nkhi, nr[RY];
-    const bool more = k < ke;
-    {
-      const double *u1 = uo + (long long)(k + 1) * sz;
-      const double *k1 = kap + (long long)(k + 1) * sz;
-      const double *u2 = u1 + sz, *k2 = k1 + sz;
-#pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        const bool p = more && lane_ld && row_ld[r];
-        nu[r] = ld(u2, off[r], p);
-        nk[r] = ld(k2, off[r], p);
-        nuxh[r] = ld(u1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
-        nkxh[r] = ld(k1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
-        if (RHS) nr[r] = ld(rhs + (long long)(k + 1) * sz, off[r], p);
-      }
-      nulo = ld(u1, lo_off, more && lane_ld);
-      nklo = ld(k1, lo_off, more && lane_ld);
-      nuhi = ld(u1, hi_off, more && lane_ld && hi_ld);
-      nkhi = ld(k1, hi_off, more && lane_ld && hi_ld);
-    }
-
-    double *out = un + (long long)k * sz;
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      const double u = uc[r], kk = kc[r];
-      const double uim = h3d_from_left(u, uxh[r]);
-      const double uip = h3d_from_right(u, uxh[r]);
-      const double kim = h3d_from_left(kk, kxh[r]);
-      const double kip = h3d_from_right(kk, kxh[r]);
-      const double ujm = (r == 0) ? ulo : uc[r - 1];
-      const double ujp = (r == RY - 1) ? uhi : uc[r + 1];
-      const double kjm = (r == 0) ? klo : kc[r - 1];
-      const double kjp = (r == RY - 1) ? khi : kc[r + 1];
-      const double s = (kip + kk) * (uip - u) - (kk + kim) * (u - uim) +
-                       (kjp + kk) * (ujp - u) - (kk + kjm) * (u - ujm) +
-                       (kq[r] + kk) * (up[r] - u) - (kk + km[r]) * (u - um[r]);
-      double res;
-      if (RHS) res = (u + rc[r]) + factor * s;
-      else res = u + factor * s;
-      if (lane_cmp && row_cmp[r]) out[off[r]] = res;
-    }
-
-#pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      um[r] = uc[r];
-      km[r] = kc[r];
-      uc[r] = up[r];
-      kc[r] = kq[r];
-      up[r] = nu[r];
-      kq[r] = nk[r];
-      uxh[r] = nuxh[r];
-      kxh[r] = nkxh[r];
-      if (RHS) rc[r] = nr[r];
-    }
-    ulo = nulo;
-    uhi = nuhi;
-    klo = nklo;
-    khi = nkhi;
-  }
+  march_f64_tile<RY, RHS>(un, uo, rhs, kap, nx, ny, sy, sz, factor, k_start,
+                          k_end, zchunk, tx, ty, tz);
 }
 
 template <int RY>
@@ -204,16 +81,18 @@ bool tune_ok(const h3d_tune &t) {
 int h3d_default_tune_f64(const h3d_grid *g, int k_start, int k_end,
                          h3d_tune *t) {
   t->rows = 2;
-  t->waves = 4;
+  t->waves = 2;
   t->kblock = 1;
-  // Enough workgroups to fill 256 CUs several times over, while keeping
-  // the re-read chunk edge planes (2 per chunk) a small fraction.
+  // March the whole z range when that already gives 2048 wavefronts (8 per
+  // CU): measured on MI355X at 512^3, the re-read edge planes of shorter
+  // chunks cost more than their extra parallelism.  Small grids split z.
   const int planes = k_end - k_start + 1;
   const long long tiles = (long long)((g->nx - 2 + 63) / 64) *
                           ((g->ny - 2 + t->rows * t->waves - 1) /
                            (t->rows * t->waves));
   int zc = planes;
-  while (zc > 32 && tiles * ((planes + zc - 1) / zc) < 4096) zc = (zc + 1) / 2;
+  while (zc > 16 && tiles * t->waves * ((planes + zc - 1) / zc) < 2048)
+    zc = (zc + 1) / 2;
   t->zchunk = zc;
   return H3D_OK;
 }

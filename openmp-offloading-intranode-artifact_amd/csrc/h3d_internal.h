@@ -29,8 +29,8 @@ static inline bool h3d_grid_ok(const h3d_grid *g) {
   if (g->nx < 3 || g->ny < 3 || g->nz < 3) return false;
   if (g->sy < g->nx) return false;
   if (g->sz < (long long)g->sy * g->ny) return false;
-  // within-plane offsets are 32-bit in the kernels
-  if (g->sz > 0x7fffffffLL) return false;
+  // within-plane byte offsets are 32-bit in the kernels (8-byte elements)
+  if (g->sz > (1LL << 28)) return false;
   return true;
 }
 

@@ -1,0 +1,444 @@
+// Tile-level device code shared by the standalone kernels (h3d_f64.hip,
+// h3d_f32.hip) and the ensemble kernel (h3d_ensemble.hip).  Including
+// translation units must be compiled with -ffp-contract=off: the FP64 tile
+// is bit-identical to the reference's update only without contraction, and
+// the FP32 tile to the CPU oracle's step-by-step FP32 arithmetic.
+#pragma once
+
+#include "h3d_internal.h"
+
+namespace {
+
+// One tile (tx, ty, tz) of the FP64 step: waves x 64 lanes, RY rows per
+// lane, planes k_start + tz*zchunk ... (see h3d_f64.hip).  Wavefronts are
+// independent (no barrier), so a wave with nothing to do returns at once.
+// Loads outside the array are predicated off rather than clamped, and the
+// x halo is loaded by lanes 0 and 63 only: on MI355X both measured faster
+// (by 4% and 7% at 512^3) than unconditional clamped loads, and plain
+// stores beat nontemporal ones by 4%.
+template <int RY, bool RHS>
+__device__ __forceinline__ void march_f64_tile(
+    double *__restrict__ un, const double *__restrict__ uo,
+    const double *__restrict__ rhs, const double *__restrict__ kap, int nx,
+    int ny, int sy, long long sz, double factor, int k_start, int k_end,
+    int zchunk, int tx, int ty, int tz) {
+#pragma clang fp contract(off)
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+
+  const int i = 1 + tx * 64 + lane;
+  const int j0 = 1 + (ty * waves + wave) * RY;
+  if (j0 > ny - 2) return;  // wave-uniform; no barriers in this function
+  const int kb = k_start + tz * zchunk;
+  if (kb > k_end) return;
+  const int ke = min(k_end, kb + zchunk - 1);
+
+  const bool lane_ld = i <= nx - 1;   // this lane's column exists
+  const bool lane_cmp = i <= nx - 2;  // and is interior
+  // x halo: lane 0 holds the point left of the segment, lane 63 the point
+  // right of it (needed only when lane 63 itself is interior).
+  const bool xh_ld = (lane == 0) || (lane == 63 && lane_cmp);
+  const int xh_di = (lane == 0) ? -1 : 1;
+
+  int off[RY];
+  bool row_ld[RY], row_cmp[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int j = j0 + r;
+    off[r] = j * sy + i;
+    row_ld[r] = j <= ny - 1;
+    row_cmp[r] = j <= ny - 2;
+  }
+  const int lo_off = (j0 - 1) * sy + i;
+  const int hi_off = (j0 + RY) * sy + i;
+  const bool hi_ld = j0 + RY <= ny - 1;
+
+  auto ld = [&](const double *p, int o, bool pred) -> double {
+    return pred ? p[o] : 0.0;
+  };
+
+  double um[RY], uc[RY], up[RY], km[RY], kc[RY], kq[RY];
+  double uxh[RY], kxh[RY], ulo, uhi, klo, khi;
+  double rc[RY];
+
+  {
+    const double *u0 = uo + (long long)(kb - 1) * sz;
+    const double *k0 = kap + (long long)(kb - 1) * sz;
+    const double *u1 = u0 + sz, *k1 = k0 + sz;
+    const double *u2 = u1 + sz, *k2 = k1 + sz;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const bool p = lane_ld && row_ld[r];
+      um[r] = ld(u0, off[r], p);
+      km[r] = ld(k0, off[r], p);
+      uc[r] = ld(u1, off[r], p);
+      kc[r] = ld(k1, off[r], p);
+      up[r] = ld(u2, off[r], p);
+      kq[r] = ld(k2, off[r], p);
+      uxh[r] = ld(u1, off[r] + xh_di, xh_ld && row_cmp[r]);
+      kxh[r] = ld(k1, off[r] + xh_di, xh_ld && row_cmp[r]);
+      if (RHS) rc[r] = ld(rhs + (long long)kb * sz, off[r], p);
+    }
+    ulo = ld(u1, lo_off, lane_ld);
+    klo = ld(k1, lo_off, lane_ld);
+    uhi = ld(u1, hi_off, lane_ld && hi_ld);
+    khi = ld(k1, hi_off, lane_ld && hi_ld);
+  }
+
+  for (int k = kb; k <= ke; ++k) {
+    // Prefetch: halos of plane k+1 and rows of plane k+2.
+    double nu[RY], nk[RY], nuxh[RY], nkxh[RY], nulo, nuhi, nklo, nkhi, nr[RY];
+    const bool more = k < ke;
+    {
+      const double *u1 = uo + (long long)(k + 1) * sz;
+      const double *k1 = kap + (long long)(k + 1) * sz;
+      const double *u2 = u1 + sz, *k2 = k1 + sz;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const bool p = more && lane_ld && row_ld[r];
+        nu[r] = ld(u2, off[r], p);
+        nk[r] = ld(k2, off[r], p);
+        nuxh[r] = ld(u1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
+        nkxh[r] = ld(k1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
+        if (RHS) nr[r] = ld(rhs + (long long)(k + 1) * sz, off[r], p);
+      }
+      nulo = ld(u1, lo_off, more && lane_ld);
+      nklo = ld(k1, lo_off, more && lane_ld);
+      nuhi = ld(u1, hi_off, more && lane_ld && hi_ld);
+      nkhi = ld(k1, hi_off, more && lane_ld && hi_ld);
+    }
+
+    double *out = un + (long long)k * sz;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const double u = uc[r], kk = kc[r];
+      const double uim = h3d_from_left(u, uxh[r]);
+      const double uip = h3d_from_right(u, uxh[r]);
+      const double kim = h3d_from_left(kk, kxh[r]);
+      const double kip = h3d_from_right(kk, kxh[r]);
+      const double ujm = (r == 0) ? ulo : uc[r - 1];
+      const double ujp = (r == RY - 1) ? uhi : uc[r + 1];
+      const double kjm = (r == 0) ? klo : kc[r - 1];
+      const double kjp = (r == RY - 1) ? khi : kc[r + 1];
+      const double s = (kip + kk) * (uip - u) - (kk + kim) * (u - uim) +
+                       (kjp + kk) * (ujp - u) - (kk + kjm) * (u - ujm) +
+                       (kq[r] + kk) * (up[r] - u) - (kk + km[r]) * (u - um[r]);
+      double res;
+      if (RHS) res = (u + rc[r]) + factor * s;
+      else res = u + factor * s;
+      if (lane_cmp && row_cmp[r]) out[off[r]] = res;
+    }
+
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      um[r] = uc[r];
+      km[r] = kc[r];
+      uc[r] = up[r];
+      kc[r] = kq[r];
+      up[r] = nu[r];
+      kq[r] = nk[r];
+      uxh[r] = nuxh[r];
+      kxh[r] = nkxh[r];
+      if (RHS) rc[r] = nr[r];
+    }
+    ulo = nulo;
+    uhi = nuhi;
+    klo = nklo;
+    khi = nkhi;
+  }
+}
+
+
+// ---- FP32 pass: see h3d_f32.hip for the temporal-blocking design ----
+
+// ulp exponent of a stored value: max(ilogb(u), -126); -126 for zero,
+// subnormals and non-finite values.
+__device__ __forceinline__ int ulp_exp(float u) {
+  const unsigned b = __float_as_uint(u) & 0x7fffffffu;
+  const int e = (int)(b >> 23) - 127;
+  return (e <= -127 || e == 128) ? -126 : e;
+}
+
+__device__ __forceinline__ unsigned short comp_encode(float e, float u) {
+  const float s = __builtin_ldexpf(e, 23 - ulp_exp(u));
+  const _Float16 h = (_Float16)s;  // round to nearest even
+  return __builtin_bit_cast(unsigned short, h);
+}
+
+__device__ __forceinline__ float comp_decode(unsigned short hb, float u) {
+  const _Float16 h = __builtin_bit_cast(_Float16, hb);
+  return __builtin_ldexpf((float)h, ulp_exp(u) - 23);
+}
+
+__device__ __forceinline__ float bf16_to_f32(unsigned short b) {
+  return __uint_as_float((unsigned)b << 16);
+}
+
+// Lane shifts for the FP32 pass.  Lanes without a source read 0: the two
+// edge lanes of a tile are never part of its valid region.
+__device__ __forceinline__ float lane_right(float v) {  // lane l <- l+1
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float lane_left(float v) {  // lane l <- l-1
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+template <typename T>
+__device__ __forceinline__ T ld_at(const void *base, unsigned byte_off) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+}
+template <typename T>
+__device__ __forceinline__ void st_at(void *base, unsigned byte_off, T v) {
+  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+}
+
+// LDS of one workgroup, in floats, for KB fused steps and W waves of RY rows:
+//   coef : [KB slots][3: cx, cy, cz][TY rows][64]  plane p in slot p mod KB
+//          (each lane reads only its own cells)
+//   cye  : [KB+1][W][64]   cy of each wave's last row, read by the wave above
+//   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
+__host__ __device__ constexpr int pass_lds_floats(int KB, int RY, int W) {
+  return KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+}
+
+template <int KB, int RY, int W, int MODE, bool BF16>
+struct Pass {
+  static constexpr int TY = W * RY;
+  static constexpr int SLOT = 3 * TY * 64;  // floats per coefficient slot
+  // ---- per-lane state ----
+  float A[KB][2][RY];        // u of level l: slot P = plane it-l-1, 1-P = it-l
+  float E[KB][2][RY];        // residual of level l, same slots
+  float FZ[KB + 1][RY];      // z flux above the last plane of level l
+  float kprev[2][RY];        // kappa rows of the previous loaded plane
+  float pu[2][RY];           // prefetched u rows (slot per phase)
+  float pk[2][RY + 1];       // prefetched kappa rows + the row above
+  unsigned short pc[2][RY];  // prefetched residual bits (COMP16)
+
+  // ---- geometry ----
+  const float *uin;
+  const float *kf;
+  const unsigned short *kb16;
+  const unsigned short *cin;
+  float *uout;
+  unsigned short *cout;
+  float factor;
+  int nz, w;
+  long long sz;
+  unsigned o4[RY + 1];  // byte offsets of rows j0..j0+RY in a float plane
+  unsigned o2[RY + 1];  // the same in a 16-bit plane
+  bool lane_int, lane_out;
+  bool row_int[RY], row_out[RY];
+  int zb, ze;
+  float *coef;  // this lane, slot 0, cx, this wave's row 0
+  float *cye;   // this lane, slot 0, wave 0
+  float *ue;    // this lane, (phase 0, level 0, wave 0, lo)
+
+  __device__ __forceinline__ int slot(int p) const {  // p mod KB
+    const int s = p % KB;
+    return s < 0 ? s + KB : s;
+  }
+  __device__ __forceinline__ int eslot(int p) const {  // p mod KB+1
+    const int s = p % (KB + 1);
+    return s < 0 ? s + KB + 1 : s;
+  }
+  __device__ __forceinline__ static constexpr int ue_at(int ph, int l, int wave, int hi) {
+    return (((ph * KB + l) * W + wave) * 2 + hi) * 64;
+  }
+
+  __device__ __forceinline__ void load_plane(int p, int ph) {
+    const int pcl = p < 0 ? 0 : (p > nz - 1 ? nz - 1 : p);
+    const long long plane = (long long)pcl * sz;
+    const float *u = uin + plane;
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      pu[ph][r] = ld_at<float>(u, o4[r]);
+      if (MODE == 2) pc[ph][r] = ld_at<unsigned short>(cin + plane, o2[r]);
+    }
+#pragma unroll
+    for (int r = 0; r <= RY; ++r) {
+      if (BF16) pk[ph][r] = bf16_to_f32(ld_at<unsigned short>(kb16 + plane, o2[r]));
+      else pk[ph][r] = ld_at<float>(kf + plane, o4[r]);
+    }
+  }
+
+  // One march iteration at phase P (P alternates; slot roles swap).
+  template <int P>
+  __device__ __forceinline__ void iter(int it) {
+#pragma clang fp contract(off)
+    // 1. Face coefficients: cz of plane it-1 now (read by level 1 below),
+    //    cx/cy of plane it at the end (their slot is read below as it-KB).
+    float cxn[RY], cyn[RY];
+    {
+      float *czs = coef + slot(it - 1) * SLOT + 2 * TY * 64;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const float k = pk[P][r];
+        cxn[r] = lane_right(k) + k;
+        cyn[r] = pk[P][r + 1] + k;
+        czs[r * 64] = kprev[1 - P][r] + k;
+        kprev[P][r] = k;
+      }
+      cye[(eslot(it) * W + w) * 64] = cyn[RY - 1];
+    }
+    // 2. Level 0: the loaded plane becomes the newest slot of level 0.
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      A[0][1 - P][r] = pu[P][r];
+      if (MODE == 2) E[0][1 - P][r] = comp_decode(pc[P][r], pu[P][r]);
+      else E[0][1 - P][r] = 0.0f;
+    }
+    // 3. Prefetch the next plane (clamped, so always a valid address).
+    load_plane(it + 1, 1 - P);
+
+    // 4. Levels 1..KB at planes it-1 .. it-KB.
+    const int wb = w > 0 ? w - 1 : 0;
+    const int wa = w < W - 1 ? w + 1 : W - 1;
+#pragma unroll
+    for (int l = 1; l <= KB; ++l) {
+      const int q = it - l;
+      const bool q_int = q >= 1 && q <= nz - 2;
+      const bool q_out = (l == KB) && q >= zb && q <= ze;
+      const float ub = ue[ue_at(1 - P, l - 1, wb, 1)];  // row below (mid plane)
+      const float ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
+      const float cyb = cye[(eslot(q) * W + wb) * 64];
+      const float *cs = coef + slot(q) * SLOT;
+      float fy_prev = 0.0f;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const float u = A[l - 1][P][r];
+        const float un = A[l - 1][1 - P][r];
+        const float ccx = cs[r * 64];
+        const float ccy = cs[TY * 64 + r * 64];
+        const float ccz = cs[2 * TY * 64 + r * 64];
+        const float fxp = ccx * (lane_right(u) - u);
+        const float fxm = lane_left(fxp);
+        const float ujp = (r < RY - 1) ? A[l - 1][P][r + 1] : ua;
+        const float fyp = ccy * (ujp - u);
+        const float fym = (r > 0) ? fy_prev : cyb * (u - ub);
+        fy_prev = fyp;
+        const float fzp = ccz * (un - u);
+        const float fzm = FZ[l][r];
+        FZ[l][r] = fzp;
+        const float s = ((((fxp - fxm) + fyp) - fym) + fzp) - fzm;
+        const float d = factor * s;
+        const bool inside = lane_int && row_int[r] && q_int;
+        float res, e2 = 0.0f;
+        if (MODE == 0) {
+          res = u + d;
+        } else {
+          // Kahan: carry the part of the increment the FP32 sum dropped
+          const float y = d + E[l - 1][P][r];
+          const float t = u + y;
+          e2 = y - (t - u);
+          res = t;
+        }
+        // Dirichlet cells keep their value; the residual of a cell is used
+        // only by that cell, so it needs no mask.
+        res = inside ? res : u;
+        if (l < KB) {
+          A[l][1 - P][r] = res;
+          E[l][1 - P][r] = e2;
+        } else if (q_out && row_out[r] && lane_out) {
+          const long long plane = (long long)q * sz;
+          st_at<float>(uout + plane, o4[r], res);
+          if (MODE == 2) st_at<unsigned short>(cout + plane, o2[r], comp_encode(e2, res));
+        }
+      }
+    }
+    // 5. Coefficients of plane it into the slot just vacated by it-KB, and
+    //    this iteration's newest edge rows of levels 0..KB-1.
+    {
+      float *cs = coef + slot(it) * SLOT;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        cs[r * 64] = cxn[r];
+        cs[TY * 64 + r * 64] = cyn[r];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < KB; ++l) {
+      ue[ue_at(P, l, w, 0)] = A[l][1 - P][0];
+      ue[ue_at(P, l, w, 1)] = A[l][1 - P][RY - 1];
+    }
+    __syncthreads();
+  }
+};
+
+// One workgroup's tile (tx, ty, tz) of an FP32 pass with W waves; `lds`
+// holds pass_lds_floats(KB, RY, W) floats.  Every thread of the workgroup
+// must call it (it synchronises the workgroup).
+template <int KB, int RY, int W, int MODE, bool BF16>
+__device__ __forceinline__ void pass_f32_tile(
+    float *lds, float *__restrict__ uout, const float *__restrict__ uin,
+    const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
+    unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
+    int nx, int ny, int nz, int sy, long long sz, float factor, int zchunk,
+    int tx, int ty, int tz) {
+  using S = Pass<KB, RY, W, MODE, BF16>;
+  S st;
+  st.uin = uin; st.kf = kf; st.kb16 = kb16; st.cin = cin;
+  st.uout = uout; st.cout = cout; st.factor = factor;
+  st.nz = nz; st.sz = sz;
+  const int lane = threadIdx.x & 63;
+  st.w = threadIdx.x >> 6;
+  constexpr int OX = 64 - 2 * KB, OY = S::TY - 2 * KB;
+
+  const int i = 1 - KB + tx * OX + lane;
+  const int y0 = 1 - KB + ty * OY;  // global y of tile row 0
+  const int j0 = y0 + st.w * RY;
+  st.zb = 1 + tz * zchunk;
+  st.ze = min(nz - 2, st.zb + zchunk - 1);
+
+  // Loads are clamped into the array: cells outside it, and the Dirichlet
+  // ring, are carried through every level unchanged, and no interior cell
+  // reads anything but interior or ring cells, so clamped values never
+  // reach a written result.
+  const int ic = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+  st.lane_int = i >= 1 && i <= nx - 2;
+  st.lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
+#pragma unroll
+  for (int r = 0; r <= RY; ++r) {
+    const int j = j0 + r;
+    const int jc = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+    const unsigned e = (unsigned)(jc * sy + ic);
+    st.o4[r] = e * 4u;
+    st.o2[r] = e * 2u;
+  }
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int j = j0 + r, tr = st.w * RY + r;
+    st.row_int[r] = j >= 1 && j <= ny - 2;
+    st.row_out[r] = tr >= KB && tr <= S::TY - 1 - KB && j <= ny - 2;
+  }
+  st.coef = lds + st.w * RY * 64 + lane;
+  st.cye = lds + KB * S::SLOT + lane;
+  st.ue = lds + KB * S::SLOT + (KB + 1) * W * 64 + lane;
+
+#pragma unroll
+  for (int l = 0; l <= KB; ++l)
+#pragma unroll
+    for (int r = 0; r < RY; ++r) st.FZ[l][r] = 0.0f;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    st.kprev[0][r] = st.kprev[1][r] = 0.0f;
+#pragma unroll
+    for (int l = 0; l < KB; ++l)
+      st.A[l][0][r] = st.A[l][1][r] = st.E[l][0][r] = st.E[l][1][r] = 0.0f;
+  }
+  // All LDS starts defined (warm-up iterations read slots not yet written).
+  for (int x = threadIdx.x; x < pass_lds_floats(KB, RY, W); x += W * 64) lds[x] = 0.0f;
+  __syncthreads();
+
+  // Iterations it0 .. it0 + 2*pairs - 1 cover zb-KB .. ze+KB; an extra
+  // trailing iteration (odd count) writes nothing (its plane is > ze).
+  const int it0 = st.zb - KB;
+  const int pairs = (st.ze + KB - it0 + 2) / 2;
+  st.load_plane(it0, 0);
+  for (int p = 0; p < pairs; ++p) {
+    st.template iter<0>(it0 + 2 * p);
+    st.template iter<1>(it0 + 2 * p + 1);
+  }
+}
+
+}  // namespace

@@ -84,7 +84,9 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   int rc = (e == hipSuccess) ? H3D_OK : (int)e;
   if (rc == H3D_OK && kind == 0) {
     const int planes = g->nz - 2;
-    const int zcs[] = {0, 256, 128, 64, 32};
+    // whole z range first: on large grids the chunk-edge re-reads cost more
+    // than the extra parallelism of short chunks buys
+    const int zcs[] = {planes, 256, 128, 64, 32};
     for (int rows : {1, 2, 4})
       for (int waves : {2, 4, 8})
         for (int zc : zcs) {

@@ -379,7 +379,7 @@ static float f16_bits_to_f32(uint16_t h)
 
 /* ulp exponent of the stored value: the residual of a cell holding u is
  * kept as e * 2^(23 - E(u)), E(u) = max(ilogb(u), -126) (-126 for 0),
- * which lies in [-0.5, 0.5] for the exact TwoSum error. */
+ * which is O(1): the Kahan residual of a sum is about half an ulp of it. */
 static inline int ulp_exp(float u)
 {
   if (u == 0.0f || !isfinite(u)) return -126;
@@ -436,12 +436,10 @@ void orc_run_f32(float *ua, float *ub, const float *kappa_f32,
             if (mode == 0) {
               un[c] = uc + d;
             } else {
-              /* TwoSum of (uc, d + e): exact error of the stored sum */
+              /* Kahan: keep the part of y the FP32 sum dropped */
               float y = d + err[c];
               float t = uc + y;
-              float bp = t - uc;
-              float ap = t - bp;
-              err[c] = (uc - ap) + (y - bp);
+              err[c] = y - (t - uc);
               un[c] = t;
             }
           }
