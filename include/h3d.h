@@ -47,7 +47,7 @@ typedef struct h3d_grid {
 typedef struct h3d_tune {
   int rows;   /* y rows per lane held in registers: 1, 2, 4 or 8     */
   int waves;  /* wavefronts per workgroup, stacked in y: 1..16        */
-  int zchunk; /* planes marched by one workgroup (0 = whole range)    */
+  int zchunk; /* planes marched by one workgroup                      */
   int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..6    */
 } h3d_tune;
 

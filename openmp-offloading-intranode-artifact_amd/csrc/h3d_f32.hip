@@ -55,14 +55,17 @@ h3d_pass_f32_kernel(float *__restrict__ uout, const float *__restrict__ uin,
                     int nz, int sy, long long sz, float factor, int zchunk,
                     int tiles_x, int tiles_y) {
   extern __shared__ float lds[];
-  unsigned t = h3d_xcd_tile(blockIdx.x, gridDim.x);
-  const int tx = (int)(t % (unsigned)tiles_x);
-  t /= (unsigned)tiles_x;
-  const int ty = (int)(t % (unsigned)tiles_y);
-  const int tz = (int)(t / (unsigned)tiles_y);
+  // one (tile, z chunk) per workgroup, z chunk slowest: the tiles of a
+  // chunk start together and march in lockstep, so the halo columns and
+  // rows they share are served from L2
+  const unsigned b = h3d_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = (int)(b % (unsigned)tiles_x);
+  const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
+  const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
+  const int zb = 1 + tz * zchunk;
   pass_f32_tile<KB, RY, W, MODE, BF16>(lds, uout, uin, kf, kb16, cout, cin, nx,
-                                       ny, nz, sy, sz, factor, zchunk, tx, ty,
-                                       tz);
+                                       ny, nz, sy, sz, factor, zb,
+                                       min(nz - 2, zb + zchunk - 1), tx, ty);
 }
 
 template <int KB, int RY, int W, int MODE, bool BF16>
@@ -75,9 +78,6 @@ int launch_one(float *uo, const float *ui, const float *kf, const uint16_t *kb,
   if (lds > 160 * 1024) return H3D_ELIMIT;
   const int tiles_x = (g->nx - 2 + OX - 1) / OX;
   const int tiles_y = (g->ny - 2 + OY - 1) / OY;
-  const int chunks = (g->nz - 2 + zchunk - 1) / zchunk;
-  const long long nb = (long long)tiles_x * tiles_y * chunks;
-  if (nb > 0x7fffffffLL) return H3D_ELIMIT;
   auto fn = h3d_pass_f32_kernel<KB, RY, W, MODE, BF16>;
   if (lds > 64 * 1024) {
     // dynamic LDS above 64 KB must be allowed per kernel, once
@@ -89,6 +89,9 @@ int launch_one(float *uo, const float *ui, const float *kf, const uint16_t *kb,
       big_lds_ok = true;
     }
   }
+  const long long nb =
+      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+  if (nb > 0x7fffffffLL) return H3D_ELIMIT;
   hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, s, uo, ui, kf,
                      (const unsigned short *)kb, (unsigned short *)co,
                      (const unsigned short *)ci, g->nx, g->ny, g->nz, g->sy,

@@ -365,16 +365,17 @@ struct Pass {
   }
 };
 
-// One workgroup's tile (tx, ty, tz) of an FP32 pass with W waves; `lds`
-// holds pass_lds_floats(KB, RY, W) floats.  Every thread of the workgroup
-// must call it (it synchronises the workgroup).
+// One workgroup's tile (tx, ty), output planes zb..ze (1 <= zb <= ze <=
+// nz-2), of an FP32 pass with W waves; `lds` holds pass_lds_floats(KB, RY,
+// W) floats.  Every thread of the workgroup must call it (it synchronises
+// the workgroup).
 template <int KB, int RY, int W, int MODE, bool BF16>
 __device__ __forceinline__ void pass_f32_tile(
     float *lds, float *__restrict__ uout, const float *__restrict__ uin,
     const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
     unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
-    int nx, int ny, int nz, int sy, long long sz, float factor, int zchunk,
-    int tx, int ty, int tz) {
+    int nx, int ny, int nz, int sy, long long sz, float factor, int zb,
+    int ze, int tx, int ty) {
   using S = Pass<KB, RY, W, MODE, BF16>;
   S st;
   st.uin = uin; st.kf = kf; st.kb16 = kb16; st.cin = cin;
@@ -387,8 +388,8 @@ __device__ __forceinline__ void pass_f32_tile(
   const int i = 1 - KB + tx * OX + lane;
   const int y0 = 1 - KB + ty * OY;  // global y of tile row 0
   const int j0 = y0 + st.w * RY;
-  st.zb = 1 + tz * zchunk;
-  st.ze = min(nz - 2, st.zb + zchunk - 1);
+  st.zb = zb;
+  st.ze = ze;
 
   // Loads are clamped into the array: cells outside it, and the Dirichlet
   // ring, are carried through every level unchanged, and no interior cell

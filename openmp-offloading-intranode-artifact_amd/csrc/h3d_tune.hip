@@ -116,7 +116,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       if (fixed_kblock && kbk != fixed_kblock) continue;
       for (int rows : {2, 4})
         for (int waves : {4, 8})
-          for (int zc : {0, 64, 128}) {
+          for (int zc : {32, 48, 64, 96, 128}) {
             h3d_tune t{rows, waves, zc, kbk};
             float ms;
             int r = time_best(s, [&] {

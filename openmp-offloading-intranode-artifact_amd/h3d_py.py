@@ -187,6 +187,40 @@ class DeviceArray:
             pass
 
 
+def pitched_grid(nx, ny, nz, align_elems=32):
+    """Grid whose row stride is a multiple of `align_elems` (32 elements =
+    128 B of FP32, 256 B of FP64), for use with AlignedField."""
+    sy = -(-nx // align_elems) * align_elems
+    return Grid(nx, ny, nz, sy, sy * ny)
+
+
+class AlignedField:
+    """Device buffer for a pitched grid, offset so that element 1 of every
+    row (the first interior point) starts a 128-byte line."""
+
+    def __init__(self, g, dtype, device=0):
+        self.dtype = np.dtype(dtype)
+        self.nbytes = g.elems * self.dtype.itemsize
+        self.buf = DeviceArray(self.nbytes + 256, device, self.dtype)
+        base = self.buf.ptr.value
+        off = (-(base + self.dtype.itemsize)) % 128
+        self.ptr = C.c_void_p(base + off)
+        self.device = device
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        call("h3d_target_memcpy", self.ptr, ptr(a), C.c_size_t(a.nbytes), self.device, -1)
+
+    def download(self):
+        out = np.empty(self.nbytes // self.dtype.itemsize, dtype=self.dtype)
+        call("h3d_target_memcpy", ptr(out), self.ptr, C.c_size_t(self.nbytes), -1, self.device)
+        return out
+
+    def close(self):
+        self.buf.close()
+
+
 class Stream:
     def __init__(self, device=0):
         p = C.c_void_p()

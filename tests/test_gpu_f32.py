@@ -65,6 +65,8 @@ CASES = [
     ((9, 75, 12), 4, 2, 8, 3),
     ((61, 61, 13), 6, 4, 4, 4),
     ((5, 5, 5), 2, 2, 4, 1),
+    ((70, 40, 30), 2, 4, 8, 29),
+    ((9, 75, 12), 1, 4, 4, 11),
 ]
 
 
