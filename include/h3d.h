@@ -177,6 +177,43 @@ int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
                  int fixed_kblock, void *stream, h3d_tune *out);
 
 /* ---------------------------------------------------------------------- */
+/* Ensembles: many independent problems of uneven sizes, one persistent
+ * launch per time step over a device-side tile work list (no reference
+ * counterpart).                                                           */
+
+typedef struct h3d_member {
+  void *u_a, *u_b;   /* device: ping-pong fields, FP64 or FP32, ring seeded */
+  const void *kappa; /* device: FP64 (precision 0) or FP32 (precision 1)    */
+  h3d_grid g;        /* member grid                                         */
+  double factor;     /* dt/h/h/2 of this member (FP32 members use it
+                        rounded to float)                                   */
+  int precision;     /* 0 = FP64 step (as h3d_step_f64); 1 = FP32 storage,
+                        compensated, with an FP16 residual field kept by the
+                        ensemble across steps (as h3d_pass_f32, one step,
+                        mode H3D_F32_COMP16)                                */
+} h3d_member;
+
+typedef struct h3d_ensemble h3d_ensemble;
+
+/* Builds the tile work list on `device_id` and allocates the FP32
+ * members' residual fields (zeroed).  tune is unused (tile shapes are
+ * fixed: 256-thread workgroups, 64-plane z chunks). */
+int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
+                        int count, const h3d_tune *tune, int device_id);
+/* `steps` time steps of every member, one launch per step.  graph != 0
+ * captures the `steps` launches into one hipGraph (kept and reused while
+ * the step count and starting parity repeat) and launches that.  *ms is
+ * the device time of the steps.  Fields ping-pong u_a -> u_b -> u_a over
+ * the ensemble's whole history: after an odd total number of steps every
+ * member's result is in u_b. */
+int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph, void *stream,
+                     float *ms);
+/* Work-list size, grid size and steps taken so far. */
+int h3d_ensemble_info(const h3d_ensemble *e, int *nwork, int *nblocks,
+                      long long *steps_done);
+int h3d_ensemble_destroy(h3d_ensemble *e);
+
+/* ---------------------------------------------------------------------- */
 /* Runtime helpers (stand-ins for omp_target_alloc / omp_target_memcpy of
  * 1-openmp.c:166-179,214).                                                */
 
