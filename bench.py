@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--dense", action="store_true",
                     help="reference's dense layout instead of 128-byte aligned rows")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the slab, 1024^3 FP32 and ensemble configurations")
     ap.add_argument("--tune64", default="", help="rows,waves,zchunk: skip FP64 autotuning")
     ap.add_argument("--tune32", default="",
                     help="kblock,rows,waves,zchunk: skip FP32 autotuning")
@@ -117,6 +119,197 @@ def cpu_baseline():
         out = dict(serial)
     out["serial"] = serial
     return out
+
+
+def seeded_problem(gx, gy, gz, kind, seed, g=None, gk0=0, member=0):
+    """Seeded fields of a (gx, gy, gz)-point problem (or of the planes of
+    grid g starting at global plane gk0) and factor = dt/h/h/2 with
+    dt = 0.9 x the positivity bound of the generated kappa."""
+    g = g or h3d.pitched_grid(gx, gy, gz)
+    u, k = h3d.gen_seeded(g, gk0, gx, gy, gz, kind, seed, member)
+    h = 1.0 / (gx - 1)
+    fmax = h3d.lib().h3d_face_sum_max_host(h3d.ptr(k), h3d.C.byref(g))
+    dt = 0.9 * 2.0 * h * h / fmax
+    return g, u, k, dt / h / h / 2.0
+
+
+def leg_cfg3(bw, stream):
+    """One GPU's slab of a 4-way z split of 1024^3: 1024 x 1024 x 256 owned
+    planes + 2 ghosts (1026 x 1026 x 258 points), lognormal kappa, 500 FP64
+    steps, boundary planes and in-HBM face pack/unpack overlapped with the
+    interior, replayed from a hipGraph.  dt from this slab's kappa."""
+    n, planes, steps = 1026, 258, 500
+    rc, nzd, gks = h3d.slab_decompose(n, 4)
+    assert rc == 0 and nzd[1] == planes
+    g = h3d.pitched_grid(n, n, planes)
+    g, u, k, factor = seeded_problem(n, n, n, 1, 20260101, g=g, gk0=gks[1] - 1)
+    fields = []
+    for x in (u, u, k):
+        f = h3d.AlignedField(g, np.float64)
+        f.upload(x)
+        fields.append(f)
+    da, db, dk = fields
+    lo, hi = h3d.DeviceArray(n * n * 8), h3d.DeviceArray(n * n * 8)
+    ms, fb = h3d.C.c_float(), h3d.C.c_int()
+    args = (da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor, lo.ptr, hi.ptr)
+    h3d.call("h3d_run_slab_f64", *args, 10, None, 1, stream.ptr, h3d.C.byref(ms),
+             h3d.C.byref(fb))
+    h3d.call("h3d_run_slab_f64", *args, steps, None, 1, stream.ptr, h3d.C.byref(ms),
+             h3d.C.byref(fb))
+    lups = (n - 2) * (n - 2) * (planes - 2) * steps
+    # per step: 2 packs + 2 unpacks, each reading and writing one plane
+    face_bytes = 4 * 2 * n * n * 8 * steps
+    gbs = (24.0 * lups + face_bytes) / (ms.value / 1e3) / 1e9
+    st = h3d.stats(db if fb.value else da, g)
+    for f in fields + [lo, hi]:
+        f.close()
+    return {"value": round(lups / (ms.value / 1e3) / 1e9, 3), "unit": "GLUP/s",
+            "dtype": "f64", "steps": steps, "ms_per_step": round(ms.value / steps, 5),
+            "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
+            "slab": [n, n, planes], "nz_split": nzd, "face_bytes": n * n * 8,
+            "nonfinite": float(st[3])}
+
+
+def leg_cfg4(bw, stream):
+    """1024^3 interior, FP32 storage, compensated, temporally blocked
+    (kblock from the autotuner), 2000 steps: kappa in FP32 and in BF16 with a
+    per-field scale, residual in registers and in an FP16 field; each against
+    the FP64 kernel run on the same (rounded / dequantised) inputs."""
+    m, steps = 1024, 2000
+    n = m + 2
+    g, u, k, factor = seeded_problem(n, n, n, 0, 20260101)
+    u32 = u.astype(np.float32)
+    del u
+    k32 = k.astype(np.float32)
+    bits, scale = h3d.quantize_bf16(k)
+    del k
+    out = {"interior": [m, m, m], "steps": steps}
+    u64 = u32.astype(np.float64)
+
+    def f64_reference(kappa64):
+        fs = []
+        for x in (u64, u64, kappa64):
+            f = h3d.AlignedField(g, np.float64)
+            f.upload(x)
+            fs.append(f)
+        ms, fb = h3d.C.c_float(), h3d.C.c_int()
+        h3d.call("h3d_run_f64", fs[0].ptr, fs[1].ptr, None, fs[2].ptr, h3d.C.byref(g),
+                 factor, steps, None, 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+        keep = fs[1] if fb.value else fs[0]
+        for f in fs:
+            if f is not keep:
+                f.close()
+        return keep, ms.value
+
+    fa, fbuf = h3d.AlignedField(g, np.float32), h3d.AlignedField(g, np.float32)
+    ca, cb = h3d.AlignedField(g, np.uint16), h3d.AlignedField(g, np.uint16)
+    for label, kap_host, kap_dtype, s in (("f32_kappa", k32, np.float32, 1.0),
+                                          ("bf16_kappa", bits, np.uint16, scale)):
+        ref, ms64 = f64_reference(k32.astype(np.float64) if s == 1.0 and kap_dtype == np.float32
+                                  else h3d.bf16_to_f64(bits, scale))
+        fk = h3d.AlignedField(g, kap_dtype)
+        fk.upload(kap_host)
+        kf, kb = (fk.ptr, None) if kap_dtype == np.float32 else (None, fk.ptr)
+        f32f = h3d.C.c_float(np.float32(factor * s))
+        for mode, mname in ((1, "carry"), (2, "fp16_residual")):
+            t = h3d.Tune()
+            h3d.call("h3d_autotune", 1, fa.ptr, fbuf.ptr, fk.ptr,
+                     1 if kb is not None else 0, h3d.C.byref(g), mode, 0, stream.ptr,
+                     h3d.C.byref(t))
+            fa.upload(u32)
+            fbuf.upload(u32)
+            zero = np.zeros(g.elems, np.uint16)
+            ca.upload(zero)
+            cb.upload(zero)
+            ms, fb = h3d.C.c_float(), h3d.C.c_int()
+            h3d.call("h3d_run_f32", fa.ptr, fbuf.ptr, kf, kb, ca.ptr, cb.ptr, h3d.C.byref(g),
+                     f32f, steps, mode, h3d.C.byref(t), 1, stream.ptr, h3d.C.byref(ms),
+                     h3d.C.byref(fb))
+            mt = h3d.compare(ref, fbuf if fb.value else fa, g, f32=True)
+            passes = -(-steps // t.kblock)
+            bpl = (8.0 if kb is None else 6.0) + 4.0 + (4.0 if mode == 2 else 0.0)
+            gbs = bpl * m ** 3 * passes / (ms.value / 1e3) / 1e9
+            out[f"{label}_{mname}"] = {
+                "value": round(m ** 3 * steps / (ms.value / 1e3) / 1e9, 3), "unit": "GLUP/s",
+                "kblock": t.kblock, "tune": [t.rows, t.waves, t.zchunk],
+                "algorithmic_bytes_per_lup": round(bpl / t.kblock, 3),
+                "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
+                "rel_l2_vs_fp64": float(mt[3]), "max_abs_vs_fp64": float(mt[0]),
+                "max_rel_vs_fp64": float(mt[4])}
+            log(f"cfg4 {label} {mname}: {out[f'{label}_{mname}']}")
+        out[f"{label}_fp64_reference_glups"] = round(m ** 3 * steps / (ms64 / 1e3) / 1e9, 3)
+        fk.close()
+        ref.close()
+    for f in (fa, fbuf, ca, cb):
+        f.close()
+    out["kappa_bf16_scale"] = scale
+    return out
+
+
+def leg_cfg5(bw, stream):
+    """64 independent problems, each axis uniform in [48, 256] interior
+    points, lognormal kappa, Gaussian u0, alternating FP64 / FP32-compensated
+    members, 500 steps captured in one hipGraph; FP32 members against their
+    FP64 twins."""
+    rng = np.random.default_rng(20260101)
+    dims = [tuple(int(x) + 2 for x in rng.integers(48, 257, 3)) for _ in range(64)]
+
+    def build(specs):
+        mems, keep = [], []
+        for s in specs:
+            dt = np.float32 if s["precision"] else np.float64
+            da, db, dk = (h3d.DeviceArray.from_host(np.ascontiguousarray(x, dtype=dt))
+                          for x in (s["u"], s["u"], s["k"]))
+            keep.append((da, db, dk))
+            mems.append(h3d.Member(da.ptr, db.ptr, dk.ptr, s["g"], s["factor"], s["precision"]))
+        arr = (h3d.Member * len(mems))(*mems)
+        e = h3d.C.c_void_p()
+        h3d.call("h3d_ensemble_create", h3d.C.byref(e), arr, len(mems), None, 0)
+        return e, keep
+
+    specs = []
+    for mi, (nx, ny, nz) in enumerate(dims):
+        g = h3d.Grid.dense(nx, ny, nz)
+        u, k = h3d.gen_seeded(g, 0, nx, ny, nz, 1, 20260101, mi)
+        prec = mi % 2
+        if prec:
+            u, k = u.astype(np.float32).astype(np.float64), k.astype(np.float32).astype(np.float64)
+        h = 1.0 / (nx - 1)
+        fmax = h3d.lib().h3d_face_sum_max_host(h3d.ptr(k), h3d.C.byref(g))
+        specs.append(dict(g=g, u=u, k=k, factor=0.9 * 2 * h * h / fmax / h / h / 2.0,
+                          precision=prec))
+    e, keep = build(specs)
+    ms = h3d.C.c_float()
+    h3d.call("h3d_ensemble_run", e, 500, 1, stream.ptr, h3d.C.byref(ms))  # builds the graph
+    h3d.call("h3d_ensemble_run", e, 500, 1, stream.ptr, h3d.C.byref(ms))  # timed: 1000 total
+    nwork, nblk, done = h3d.C.c_int(), h3d.C.c_int(), h3d.C.c_longlong()
+    h3d.call("h3d_ensemble_info", e, h3d.C.byref(nwork), h3d.C.byref(nblk), h3d.C.byref(done))
+    cells = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in dims)
+    sec = ms.value / 1e3
+    glups = cells * 500 / sec / 1e9
+    twins = [dict(s, precision=0) for s in specs[1::2]]
+    e2, keep2 = build(twins)
+    h3d.call("h3d_ensemble_run", e2, 1000, 1, stream.ptr, h3d.C.byref(ms))
+    worst = 0.0
+    for (da, db, dk), (ta, tb, tk) in zip(keep[1::2], keep2):
+        got = da.download().astype(np.float64)  # 1000 steps: results in u_a
+        ref = ta.download()
+        worst = max(worst, float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
+    h3d.call("h3d_ensemble_destroy", e)
+    h3d.call("h3d_ensemble_destroy", e2)
+    for trip in keep + keep2:
+        for d in trip:
+            d.close()
+    # FP64 members 24 B per update; FP32 members 4+4+4 B plus 2+2 B residual
+    b64 = sum(24.0 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
+    b32 = sum(16.0 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
+    gbs = (b64 + b32) * 500 / sec / 1e9
+    return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
+            "cells": int(cells), "steps_timed": 500, "graph": True,
+            "ms_per_step": round(1e3 * sec / 500, 5),
+            "work_items": nwork.value, "workgroups": nblk.value,
+            "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
+            "worst_fp32_member_rel_l2_vs_fp64": worst}
 
 
 def main():
@@ -269,6 +462,24 @@ def main():
                         "speedup>=1.8": bool(glups32 / glups >= 1.8),
                         "rel_l2<=1e-5": bool(m32[3] <= 1e-5)},
         }
+
+    # ---- the other configurations, each reported beside the headline ----
+    for d in (da, db, dk):
+        d.close()
+    if not a.no_fp32:
+        for d in (fa, fbuf, fk):
+            d.close()
+    if not a.no_configs:
+        result["configs"] = {}
+        for name, fn in (("slab_1024x1024x256_fp64", leg_cfg3),
+                         ("fp32c_1024cubed_2000steps", leg_cfg4),
+                         ("ensemble_64_members_500steps", leg_cfg5)):
+            log(f"{name} ...")
+            try:
+                result["configs"][name] = fn(bw, stream)
+            except Exception as exc:  # an extra leg never hides the headline
+                result["configs"][name] = {"error": f"{type(exc).__name__}: {exc}"}
+            log(f"{name}: {json.dumps(result['configs'][name])[:300]}")
 
     # ---- CPU baseline ----
     if not a.no_cpu:

@@ -111,6 +111,27 @@ int h3d_pack_face_f32(float *buf, const float *u, const h3d_grid *g, int k,
 int h3d_unpack_face_f32(float *u, const float *buf, const h3d_grid *g, int k,
                         void *stream);
 
+/* One step of one z-slab (nz >= 4 planes: 2 ghosts + owned planes) in the
+ * shape of the reference's middle devices (4-openmp-stream-omp-p2p.c:
+ * 655-727): boundary planes 1 and nz-2, interior 2..nz-3, then the faces
+ * pushed into the ghost planes of u_new through face_lo / face_hi (device
+ * buffers of nx*ny values).  With one GPU the slab is its own neighbour:
+ * plane nz-2 lands in ghost 0 and plane 1 in ghost nz-1 (z-periodic
+ * stand-in for the peer exchange, same kernels and bytes). */
+int h3d_slab_step_f64(double *u_new, const double *u_old, const double *rhs,
+                      const double *kappa, const h3d_grid *g, double factor,
+                      double *face_lo, double *face_hi, const h3d_tune *tune,
+                      void *stream);
+/* `steps` slab steps with u_a/u_b ping-pong; the interior kernel runs on
+ * a second stream overlapped with the boundary planes and face copies (as
+ * the reference overlaps its interior kernel with the exchange); graph != 0
+ * replays a captured pair of steps.  stream must be a created stream. */
+int h3d_run_slab_f64(double *u_a, double *u_b, const double *rhs,
+                     const double *kappa, const h3d_grid *g, double factor,
+                     double *face_lo, double *face_hi, int steps,
+                     const h3d_tune *tune, int graph, void *stream, float *ms,
+                     int *final_in_b);
+
 /* FP64 diagnostics over the whole array (ring included), deterministic:
  * out[0] = sum u (checksum), out[1] = sum u^2, out[2] = max |u|,
  * out[3] = number of non-finite values.  Host `out`; synchronises stream. */

@@ -351,25 +351,20 @@ int main(int argc, char **argv)
   double timer = -wtime();
   int in_b = 0;
   float dev_ms = 0.0f;
-  if (!f32 && !o.slab_parts && o.graph) {
+  if (!f32 && o.slab_parts) {
+    /* middle-slab shape (2-openmp.c:25-55): boundary planes and the face
+     * exchange overlapped with the interior on a second stream */
+    CHECK(h3d_run_slab_f64(d_ua, d_ub, d_rhs, d_kap, &g, factor, d_face_lo,
+                           d_face_hi, steps, &tune, o.graph, stream, &dev_ms,
+                           &in_b));
+  } else if (!f32 && o.graph) {
     CHECK(h3d_run_f64(d_ua, d_ub, d_rhs, d_kap, &g, factor, steps, &tune, 1,
                       stream, &dev_ms, &in_b));
   } else if (!f32) {
+    /* the reference's loop (1-openmp.c:193-205): one update, then swap */
     double *uo = d_ua, *un = d_ub;
     for (int s = 0; s < steps; s++) {
-      if (o.slab_parts) {
-        /* middle-slab shape: bottom plane, top plane, then the interior
-         * (2-openmp.c:25-55); faces of u_new refreshed afterwards */
-        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 1, 1, &tune, stream));
-        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, nz - 2, nz - 2, &tune, stream));
-        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 2, nz - 3, &tune, stream));
-        CHECK(h3d_pack_face_f64(d_face_lo, un, &g, 1, stream));
-        CHECK(h3d_pack_face_f64(d_face_hi, un, &g, nz - 2, stream));
-        CHECK(h3d_unpack_face_f64(un, d_face_hi, &g, 0, stream));
-        CHECK(h3d_unpack_face_f64(un, d_face_lo, &g, nz - 1, stream));
-      } else {
-        CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 1, nz - 2, &tune, stream));
-      }
+      CHECK(h3d_step_f64(un, uo, d_rhs, d_kap, &g, factor, 1, nz - 2, &tune, stream));
       double *t = uo; uo = un; un = t;
     }
     in_b = (uo == d_ub);

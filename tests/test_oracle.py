@@ -265,6 +265,67 @@ def test_f32_plain_matches_numpy(oracle):
     assert np.array_equal(got, want)
 
 
+def np_ulp_exp(u):
+    """max(ilogb(u), -126), -126 for 0 (numpy float32 restatement)."""
+    m, e = np.frexp(u.astype(np.float32))
+    e = (e - 1).astype(np.int32)
+    e[(u == 0) | ~np.isfinite(u)] = -126
+    return np.maximum(e, -126)
+
+
+def numpy_f32_compensated(u, k, factor, steps, kb, mode, shape):
+    """Kahan-compensated FP32 steps with the residual reset per pass of kb
+    steps (mode 1) or kept across passes as FP16 scaled by the ulp (mode 2)."""
+    u = u.reshape(shape).astype(np.float32).copy()
+    k = k.reshape(shape)
+    f = np.float32(factor)
+    comp = np.zeros(shape, np.float16)
+    done = 0
+    inner = (slice(1, -1),) * 3
+    while done < steps:
+        ns = min(kb, steps - done)
+        if mode == 2:
+            e = np.ldexp(comp.astype(np.float32), np_ulp_exp(u) - 23).astype(np.float32)
+        else:
+            e = np.zeros(shape, np.float32)
+        def nb(a, dz, dy, dx):
+            n0, n1, n2 = a.shape
+            return a[1 + dz:n0 - 1 + dz, 1 + dy:n1 - 1 + dy, 1 + dx:n2 - 1 + dx]
+
+        for _ in range(ns):
+            c = u[inner]
+            kc = k[inner]  # S in float32, in the reference's term order
+            S = ((nb(k, 0, 0, 1) + kc) * (nb(u, 0, 0, 1) - c)
+                 - (kc + nb(k, 0, 0, -1)) * (c - nb(u, 0, 0, -1))
+                 + (nb(k, 0, 1, 0) + kc) * (nb(u, 0, 1, 0) - c)
+                 - (kc + nb(k, 0, -1, 0)) * (c - nb(u, 0, -1, 0))
+                 + (nb(k, 1, 0, 0) + kc) * (nb(u, 1, 0, 0) - c)
+                 - (kc + nb(k, -1, 0, 0)) * (c - nb(u, -1, 0, 0)))
+            d = f * S
+            y = d + e[inner]
+            t = c + y
+            e[inner] = y - (t - c)
+            u = u.copy()
+            u[inner] = t
+        if mode == 2:
+            sc = np.ldexp(e, 23 - np_ulp_exp(u)).astype(np.float32)
+            comp = sc.astype(np.float16)
+        done += ns
+    return u.ravel()
+
+
+@pytest.mark.parametrize("mode,kb", [(1, 1), (1, 3), (2, 1), (2, 4)])
+def test_f32_compensated_matches_numpy(oracle, mode, kb):
+    nx, ny, nz = 10, 9, 8
+    u, k = f32_problem(nx, ny, nz, seed=6)
+    factor = 0.05
+    steps = 2 * kb + 1
+    got, _ = oracle.run_f32(u, k, None, nx, ny, nz, factor, steps, kb, mode)
+    with np.errstate(all="ignore"):
+        want = numpy_f32_compensated(u, k, factor, steps, kb, mode, (nz, ny, nx))
+    assert np.array_equal(got, want)
+
+
 def test_f32_compensated_is_closer_to_f64(oracle):
     nx, ny, nz = 12, 11, 10
     u, k = f32_problem(nx, ny, nz, seed=9)
