@@ -60,7 +60,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   if (!u_a || !kappa || !out || !h3d_grid_ok(g)) return H3D_EARG;
   if (kind != 0 && kind != 1) return H3D_EARG;
   if (kind == 1 && (mode < 0 || mode > 2)) return H3D_EARG;
-  if (fixed_kblock < 0 || fixed_kblock > 8) return H3D_ELIMIT;
+  if (fixed_kblock < 0 || fixed_kblock > 6) return H3D_ELIMIT;
   const Key key{kind, g->nx, g->ny, g->nz, g->sy, kind ? mode : 0,
                 kind ? kappa_bf16 : 0, kind ? fixed_kblock : 0, g->sz};
   {

@@ -53,6 +53,30 @@ def test_oracle_openmp_equals_serial(oracle, entry):
     assert np.array_equal(a, b)
 
 
+REF_SERIAL = os.path.join(os.path.dirname(GOLDEN), "..", "oracle", "_ref", "serial")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SERIAL),
+                    reason="reference serial program not built (needs the reference tree)")
+@pytest.mark.parametrize("n,steps", [(24, 13), (41, 6), (50, 3)])
+def test_oracle_matches_reference_binary_live(oracle, tmp_path, n, steps):
+    """Beyond the committed fixtures: the reference's serial program built
+    from source (oracle/Makefile `ref`), at sizes with odd and even n and a
+    step count from the %.7g T rule, against the oracle, byte for byte."""
+    import subprocess
+    h, dt, factor = oracle.params_reference(n)
+    T = "%.7g" % (steps * dt)
+    env = dict(os.environ, WRITE_TXT="1", OMP_NUM_THREADS="2")
+    out = subprocess.run([os.path.abspath(REF_SERIAL), str(n), T], cwd=tmp_path, env=env,
+                         check=True, capture_output=True, text=True, timeout=120)
+    recorded = int(next(ln for ln in out.stdout.splitlines()
+                        if ln.endswith("timesteps complete")).split()[0])
+    assert recorded == oracle.count_steps(float(T), dt)
+    u, un, rhs, k = oracle.init_reference(n)
+    res = oracle.run_f64(u, un, rhs, k, n, n, n, recorded, factor=factor)
+    assert open(tmp_path / "u_serial.txt").read().splitlines() == dump_lines(res)
+
+
 def test_dump_format_roundtrip(oracle, tmp_path):
     entry = INDEX[-1]
     n = entry["n"]
