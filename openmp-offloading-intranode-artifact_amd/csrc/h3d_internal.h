@@ -41,11 +41,13 @@ static inline size_t h3d_grid_elems(const h3d_grid *g) {
 // XCD-aware tile order.  Workgroups are dealt round-robin to the 8 XCDs
 // (workgroup b runs on XCD b % 8), each with its own L2.  Remapping so
 // that XCD x receives a contiguous run of logical tiles keeps neighbouring
-// tiles, which share halo rows, in one L2.
+// tiles, which share halo rows, in one L2.  XCD x gets q+1 workgroups if
+// x < r (nb = 8q + r), else q; its run starts after those of XCDs 0..x-1,
+// so the map is a bijection of [0, nb) for every nb.
 __device__ __forceinline__ unsigned h3d_xcd_tile(unsigned b, unsigned nb) {
-  const unsigned nxcd = 8;
-  if (nb % nxcd) return b;
-  return (b % nxcd) * (nb / nxcd) + b / nxcd;
+  const unsigned nxcd = 8, q = nb / nxcd, r = nb % nxcd;
+  const unsigned x = b % nxcd, p = b / nxcd;
+  return x * q + (x < r ? x : r) + p;
 }
 
 // Whole-wavefront lane shifts through DPP (wave_shr:1 / wave_shl:1): lane

@@ -184,13 +184,29 @@ __device__ __forceinline__ float lane_left(float v) {  // lane l <- l-1
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
-template <typename T>
-__device__ __forceinline__ T ld_at(const void *base, unsigned byte_off) {
-  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + byte_off);
+// Plane access through raw buffer instructions: one resource per plane
+// (base = the plane's first element, range = its bytes, built in scalar
+// registers) and a 32-bit per-lane byte offset, so a load or store costs
+// no 64-bit vector address arithmetic.  Offsets are always inside the
+// plane (loads are clamped, stores predicated); the range check of the
+// resource would turn a stray access into a zero read / dropped write.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base,
+                                                            unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes,
+                                           0x00020000);
 }
-template <typename T>
-__device__ __forceinline__ void st_at(void *base, unsigned byte_off, T v) {
-  *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + byte_off) = v;
+__device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ unsigned short ld_u16(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void st_f32(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)off, 0, 0);
+}
+__device__ __forceinline__ void st_u16(__amdgpu_buffer_rsrc_t r, unsigned off,
+                                       unsigned short v) {
+  __builtin_amdgcn_raw_buffer_store_b16(v, r, (int)off, 0, 0);
 }
 
 // LDS of one workgroup, in floats, for KB fused steps and W waves of RY rows:
@@ -227,8 +243,10 @@ struct Pass {
   long long sz;
   unsigned o4[RY + 1];  // byte offsets of rows j0..j0+RY in a float plane
   unsigned o2[RY + 1];  // the same in a 16-bit plane
-  bool lane_int, lane_out;
-  bool row_int[RY], row_out[RY];
+  unsigned s4[RY];      // store offsets: o4, or OUT_OF_RANGE off the output region
+  unsigned s2[RY];      // the same for the 16-bit residual plane
+  bool lane_int;
+  bool row_int[RY];
   int zb, ze;
   float *coef;  // this lane, slot 0, cx, this wave's row 0
   float *cye;   // this lane, slot 0, wave 0
@@ -249,16 +267,17 @@ struct Pass {
   __device__ __forceinline__ void load_plane(int p, int ph) {
     const int pcl = p < 0 ? 0 : (p > nz - 1 ? nz - 1 : p);
     const long long plane = (long long)pcl * sz;
-    const float *u = uin + plane;
+    const unsigned b4 = (unsigned)sz * 4u, b2 = (unsigned)sz * 2u;
+    const auto ru = plane_rsrc(uin + plane, b4);
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      pu[ph][r] = ld_at<float>(u, o4[r]);
-      if (MODE == 2) pc[ph][r] = ld_at<unsigned short>(cin + plane, o2[r]);
+      pu[ph][r] = ld_f32(ru, o4[r]);
+      if (MODE == 2) pc[ph][r] = ld_u16(plane_rsrc(cin + plane, b2), o2[r]);
     }
 #pragma unroll
     for (int r = 0; r <= RY; ++r) {
-      if (BF16) pk[ph][r] = bf16_to_f32(ld_at<unsigned short>(kb16 + plane, o2[r]));
-      else pk[ph][r] = ld_at<float>(kf + plane, o4[r]);
+      if (BF16) pk[ph][r] = bf16_to_f32(ld_u16(plane_rsrc(kb16 + plane, b2), o2[r]));
+      else pk[ph][r] = ld_f32(plane_rsrc(kf + plane, b4), o4[r]);
     }
   }
 
@@ -303,6 +322,9 @@ struct Pass {
       const float ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
       const float cyb = cye[(eslot(q) * W + wb) * 64];
       const float *cs = coef + slot(q) * SLOT;
+      const long long qplane = (long long)(q_out ? q : zb) * sz;
+      const auto ro = plane_rsrc(uout + qplane, (unsigned)sz * 4u);
+      const auto rc = plane_rsrc(MODE == 2 ? cout + qplane : nullptr, (unsigned)sz * 2u);
       float fy_prev = 0.0f;
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
@@ -333,16 +355,18 @@ struct Pass {
           e2 = y - (t - u);
           res = t;
         }
-        // Dirichlet cells keep their value; the residual of a cell is used
-        // only by that cell, so it needs no mask.
-        res = inside ? res : u;
         if (l < KB) {
-          A[l][1 - P][r] = res;
+          // Dirichlet cells keep their value; the residual of a cell is used
+          // only by that cell, so it needs no mask.
+          A[l][1 - P][r] = inside ? res : u;
           E[l][1 - P][r] = e2;
-        } else if (q_out && row_out[r] && lane_out) {
-          const long long plane = (long long)q * sz;
-          st_at<float>(uout + plane, o4[r], res);
-          if (MODE == 2) st_at<unsigned short>(cout + plane, o2[r], comp_encode(e2, res));
+        } else if (q_out) {
+          // Stored cells are interior on every axis, so the last level needs
+          // no Dirichlet select; rows and lanes off the output region carry
+          // an out-of-range offset and the buffer range check drops them.
+          // The branch is uniform (q is the same for the whole workgroup).
+          st_f32(ro, s4[r], res);
+          if (MODE == 2) st_u16(rc, s2[r], comp_encode(e2, res));
         }
       }
     }
@@ -397,7 +421,7 @@ __device__ __forceinline__ void pass_f32_tile(
   // reach a written result.
   const int ic = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
   st.lane_int = i >= 1 && i <= nx - 2;
-  st.lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
+  const bool lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
 #pragma unroll
   for (int r = 0; r <= RY; ++r) {
     const int j = j0 + r;
@@ -406,11 +430,16 @@ __device__ __forceinline__ void pass_f32_tile(
     st.o4[r] = e * 4u;
     st.o2[r] = e * 2u;
   }
+  // Planes are at most 2^28 elements (1 GiB of floats), so 2^31 is past
+  // the range of every plane resource.
+  constexpr unsigned OUT_OF_RANGE = 0x80000000u;
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r, tr = st.w * RY + r;
     st.row_int[r] = j >= 1 && j <= ny - 2;
-    st.row_out[r] = tr >= KB && tr <= S::TY - 1 - KB && j <= ny - 2;
+    const bool out = lane_out && tr >= KB && tr <= S::TY - 1 - KB && j <= ny - 2;
+    st.s4[r] = out ? st.o4[r] : OUT_OF_RANGE;
+    st.s2[r] = out ? st.o2[r] : OUT_OF_RANGE;
   }
   st.coef = lds + st.w * RY * 64 + lane;
   st.cye = lds + KB * S::SLOT + lane;
