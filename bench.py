@@ -330,6 +330,14 @@ def main():
     copy_gbs, triad_gbs = h3d.stream_probe(4 << 30, reps=5)
     bw = max(copy_gbs, triad_gbs)
     log(f"stream: copy {copy_gbs:.0f} GB/s, triad {triad_gbs:.0f} GB/s")
+    # host <-> device rates (the boundary takes device buffers, so these are
+    # reported beside the headline, never folded into it)
+    transfer = {}
+    for kind, pinned in (("pinned", True), ("pageable", False)):
+        h2d, d2h, lh, ld = h3d.transfer_probe(64 << 20, reps=3, pinned=pinned)
+        transfer[kind] = {"h2d_gbs": round(h2d, 2), "d2h_gbs": round(d2h, 2),
+                          "h2d_4k_us": round(lh, 2), "d2h_4k_us": round(ld, 2)}
+    log(f"transfer: {transfer}")
 
     # ---- problem ----
     m = a.m
@@ -405,6 +413,7 @@ def main():
                      "kernel_ms_per_launch": round(1e3 * per_launch_s, 5)},
         "stream": {"copy_gbs": round(copy_gbs, 1), "triad_gbs": round(triad_gbs, 1),
                    "array_bytes": 4 << 30},
+        "transfer": dict(transfer, bytes=64 << 20),
         "checksum": {"sum": float(st[0]), "sumsq": float(st[1]), "max": float(st[2])},
     }
 

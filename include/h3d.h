@@ -259,6 +259,15 @@ int h3d_device_arch(int device_id, char *buf, int buflen);
 int h3d_stream_probe(size_t bytes, int reps, int device_id, double *copy_gbs,
                      double *triad_gbs);
 
+/* Host <-> device copy rates, as the reference's transfer benchmarks
+ * measure them (Bandwidth_Measurements/AMD-MI250X/Synchronous-test-H-to-D-
+ * Measurements/hip-1.hip:56-92, and the D-to-H / asynchronous twins): host
+ * clock around blocking copies, best of `reps`.  out[0] H2D GB/s, out[1]
+ * D2H GB/s for `bytes` (>= 4096); out[2] H2D, out[3] D2H microseconds of a
+ * 4 KiB copy.  pinned != 0: hipHostMalloc'ed host buffer, else malloc'ed. */
+int h3d_transfer_probe(size_t bytes, int reps, int pinned, int device_id,
+                       double *out);
+
 /* ---------------------------------------------------------------------- */
 /* Host-side problem set-up (plain C; no device involved).                 */
 

@@ -240,3 +240,56 @@ extern "C" int h3d_stream_probe(size_t bytes, int reps, int device_id,
   if (triad_gbs) *triad_gbs = best_triad;
   return H3D_OK;
 }
+
+// Host <-> device copy rates: the quantities of the reference's transfer
+// benchmarks (Bandwidth_Measurements/AMD-MI250X/Synchronous-test-H-to-D-
+// Measurements/hip-1.hip:56-92 and its D-to-H and asynchronous twins),
+// timed the same way, with the host clock around a blocking copy after two
+// warm-up copies.  Best of `reps`.  out[0] H2D GB/s and out[1] D2H GB/s
+// for `bytes`; out[2] H2D and out[3] D2H microseconds of a 4 KiB copy
+// (the fixed cost of one transfer).  pinned != 0 uses hipHostMalloc'ed
+// host memory, else malloc'ed (pageable) memory.
+#include <chrono>
+#include <stdlib.h>
+
+extern "C" int h3d_transfer_probe(size_t bytes, int reps, int pinned, int device_id,
+                                  double *out) {
+  if (bytes < 4096 || reps < 1 || !out) return H3D_EARG;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  void *h = nullptr, *d = nullptr;
+  hipError_t e = pinned ? hipHostMalloc(&h, bytes, hipHostMallocDefault)
+                        : (h = malloc(bytes)) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess) e = hipMalloc(&d, bytes);
+  if (e == hipSuccess) memset(h, 0x3c, bytes);
+  using clk = std::chrono::steady_clock;
+  auto best = [&](hipMemcpyKind kind, size_t nb, int n, double *sec) {
+    void *dst = kind == hipMemcpyHostToDevice ? d : h;
+    const void *src = kind == hipMemcpyHostToDevice ? h : d;
+    for (int w = 0; w < 2 && e == hipSuccess; ++w) e = hipMemcpy(dst, src, nb, kind);
+    double b = 1e30;
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
+      const auto t0 = clk::now();
+      for (int i = 0; i < n && e == hipSuccess; ++i) e = hipMemcpy(dst, src, nb, kind);
+      const double s = std::chrono::duration<double>(clk::now() - t0).count() / n;
+      if (s < b) b = s;
+    }
+    *sec = b;
+  };
+  double s[4] = {0, 0, 0, 0};
+  best(hipMemcpyHostToDevice, bytes, 1, &s[0]);
+  best(hipMemcpyDeviceToHost, bytes, 1, &s[1]);
+  best(hipMemcpyHostToDevice, 4096, 100, &s[2]);
+  best(hipMemcpyDeviceToHost, 4096, 100, &s[3]);
+  if (d) (void)hipFree(d);
+  if (h) {
+    if (pinned) (void)hipHostFree(h);
+    else free(h);
+  }
+  if (e != hipSuccess) return (int)e;
+  out[0] = bytes / s[0] / 1e9;
+  out[1] = bytes / s[1] / 1e9;
+  out[2] = s[2] * 1e6;
+  out[3] = s[3] * 1e6;
+  return H3D_OK;
+}

@@ -90,6 +90,7 @@ SIGNATURES = {
     "h3d_error_string": (C.c_char_p, [_I]),
     "h3d_device_arch": (_I, [_I, C.c_char_p, _I]),
     "h3d_stream_probe": (_I, [C.c_size_t, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
+    "h3d_transfer_probe": (_I, [C.c_size_t, _I, _I, _I, _P]),
     "h3d_params_reference": (_I, [_I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
     "h3d_count_steps": (_I, [_D, _D]),
     "h3d_init_reference": (_I, [_P, _P, _P, _GP]),
@@ -312,3 +313,10 @@ def stream_probe(nbytes, reps=5, device=0):
     c, t = C.c_double(), C.c_double()
     call("h3d_stream_probe", C.c_size_t(nbytes), reps, device, C.byref(c), C.byref(t))
     return c.value, t.value
+
+
+def transfer_probe(nbytes, reps=3, pinned=True, device=0):
+    """(H2D GB/s, D2H GB/s, H2D us, D2H us) of host<->device copies."""
+    out = np.zeros(4)
+    call("h3d_transfer_probe", C.c_size_t(nbytes), reps, int(pinned), device, ptr(out))
+    return tuple(float(x) for x in out)

@@ -78,3 +78,65 @@ def test_cli_autotune(tmp_path):
     out = run(["70", "--steps", "4", "--mode", "seeded", "--autotune"], str(tmp_path),
               write=False)
     assert re.search(r"^Autotuned: rows=\d+ waves=\d+ zchunk=\d+", out, re.M)
+
+
+VERIFY = os.path.join(REPO, "openmp-offloading-intranode-artifact_amd", "bin", "h3d-verify")
+
+
+def verify(args, cwd):
+    return subprocess.run([VERIFY] + args, cwd=cwd, capture_output=True, text=True,
+                          timeout=120)
+
+
+def compare_line(label, name, m):
+    """The metrics line of compare.py:101-103 for oracle metrics m."""
+    ok = m[6] == 0 and m[7] == 0 and m[8] == 0
+    return (f"  [{'PASS' if ok else 'FAIL'} ] {label:<22} {name:<34} "
+            f"maxabs={m[0]:.3e}  rms={m[2]:.3e}  relL2={m[3]:.3e}  maxrel={m[4]:.3e}")
+
+
+def test_verify_passes_cli_dump_against_reference_dump(tmp_path):
+    """The pipeline's verify step: the reference serial program's dump as
+    u_serial.txt, h3d's own dump found by name, bit-identical -> PASS."""
+    entry = next(e for e in INDEX if e["n"] == 33)
+    with gzip.open(os.path.join(GOLDEN, entry["file"]), "rt") as f:
+        (tmp_path / "u_serial.txt").write_text(f.read())
+    run([str(entry["n"]), entry["T"]], str(tmp_path))
+    r = verify([], str(tmp_path))
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert f"Reference: u_serial.txt  ({33 ** 3} values)" in r.stdout
+    assert "Tolerance: atol=1e-06, rtol=1e-05" in r.stdout
+    assert ("  [PASS ] 1-GPU h3d              u_h3d_1gpu.txt                     "
+            "maxabs=0.000e+00  rms=0.000e+00  relL2=0.000e+00  maxrel=0.000e+00") in r.stdout
+    assert "Summary: 1/1 passed (atol=1e-06, rtol=1e-05)" in r.stdout
+
+
+def test_verify_metrics_and_failures_match_oracle(oracle, tmp_path):
+    n = 17
+    rng = np.random.default_rng(5)
+    ref = rng.standard_normal(n ** 3)
+    ref[::40] = 0.0
+    good = ref * (1 + 1e-7 * rng.standard_normal(ref.size))
+    bad = ref + 1e-3 * rng.standard_normal(ref.size)
+    nan = good.copy()
+    nan[100] = np.nan
+    for name, a in (("ref.txt", ref), ("good.txt", good), ("bad.txt", bad), ("nan.txt", nan),
+                    ("short.txt", ref[:-1])):
+        (tmp_path / name).write_text("".join("%.17g\n" % v for v in a))
+    r = verify(["--ref", "ref.txt", "--atol", "1e-6", "--rtol", "1e-5", "good.txt",
+                "bad.txt", "nan.txt", "short.txt", "missing.txt"], str(tmp_path))
+    assert r.returncode == 1, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    for name, a in (("good.txt", good), ("bad.txt", bad), ("nan.txt", nan)):
+        m = oracle.metrics(ref, a, 1e-6, 1e-5)
+        assert compare_line(name, name, m) in lines, (name, r.stdout)
+    m = oracle.metrics(ref, bad, 1e-6, 1e-5)
+    i = int(m[5])
+    assert f"           worst @ idx {i}: ref={ref[i]:.6e}  cand={bad[i]:.6e}" in lines
+    assert any(ln.endswith("NaN=1 Inf=0") for ln in lines)
+    assert (f"  [FAIL ] {'short.txt':<22} {'short.txt':<34} size mismatch: "
+            f"ref={n ** 3} cand={n ** 3 - 1}") in lines
+    assert "Summary: 1/4 passed (atol=1e-06, rtol=1e-05)" in r.stdout
+    # nothing to compare -> 2
+    r = verify(["--ref", "ref.txt", "missing.txt"], str(tmp_path))
+    assert r.returncode == 2 and "no candidate GPU output files found" in r.stdout

@@ -205,3 +205,12 @@ def test_face_sum_max_device_matches_host(gpu, oracle):
 def test_stream_probe_runs(gpu):
     c, t = gpu.stream_probe(1 << 28, reps=2)
     assert c > 100.0 and t > 100.0 and math.isfinite(c)
+
+
+@pytest.mark.parametrize("pinned", [True, False])
+def test_transfer_probe_runs(gpu, pinned):
+    h2d, d2h, lat_h2d, lat_d2h = gpu.transfer_probe(1 << 26, reps=2, pinned=pinned)
+    assert h2d > 0.5 and d2h > 0.5 and 0 < lat_h2d < 1e4 and 0 < lat_d2h < 1e4
+    L = gpu.lib()
+    out = np.zeros(4)
+    assert L.h3d_transfer_probe(100, 1, 1, 0, gpu.ptr(out)) == -1

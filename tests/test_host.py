@@ -190,3 +190,17 @@ def test_cli_rejects_bad_arguments():
     r = subprocess.run([exe, "9", "--slab", "4:0", "--steps", "1"], capture_output=True,
                        text=True, env=dict(os.environ, WRITE_TXT="0"))
     assert r.returncode == 1 and "cannot be split" in r.stderr
+
+
+def test_verify_tool_exit_codes_without_device(tmp_path):
+    """compare.py's exit codes (compare.py:120-151): 2 without a reference
+    file, 2 on a bad option; nothing here touches a GPU."""
+    exe = os.path.join(PKG, "bin", "h3d-verify")
+    r = subprocess.run([exe], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2
+    assert "ERROR: reference file 'u_serial.txt' not found." in r.stdout
+    r = subprocess.run([exe, "--tolerance", "1"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+    (tmp_path / "r.txt").write_text("1\n2\n")
+    r = subprocess.run([exe, "--ref", "r.txt"], cwd=tmp_path, capture_output=True, text=True)
+    assert r.returncode == 2 and "not n^3" in r.stdout
