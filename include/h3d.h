@@ -43,10 +43,14 @@ typedef struct h3d_grid {
 } h3d_grid;
 
 /* Launch shape of the stencil kernels.  Zero fields take the library's
- * defaults (or the autotuner's cached choice). */
+ * defaults (or the autotuner's cached choice).  Supported shapes (others
+ * return H3D_EARG):
+ *   FP64 step and FP32 single-step pass: rows 1, 2, 4 (FP64 also 8),
+ *     waves 1..16 for FP64, 1..8 for FP32;
+ *   FP32 fused passes (kblock 2..6): rows 2 or 4, waves 4 or 8. */
 typedef struct h3d_tune {
-  int rows;   /* y rows per lane held in registers: 1, 2, 4 or 8     */
-  int waves;  /* wavefronts per workgroup, stacked in y: 1..16        */
+  int rows;   /* y rows per lane held in registers                   */
+  int waves;  /* wavefronts per workgroup, stacked in y               */
   int zchunk; /* planes marched by one workgroup                      */
   int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..6    */
 } h3d_tune;

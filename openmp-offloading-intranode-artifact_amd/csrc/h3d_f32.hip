@@ -41,6 +41,13 @@
 // Residual e of each cell is carried through the KB steps in registers; in
 // COMP16 mode it is also read at the start of the pass and written at the
 // end as FP16 scaled by the cell's ulp, 2^(23 - E(u)).
+//
+// Single-step passes (KB = 1) need no level sharing, so they run a
+// separate wave-independent tile (march_f32_tile): 64 x (waves * RY) cells
+// per workgroup with no overlap between tiles, no LDS and no barrier, the
+// x/y halos loaded directly.  It computes the same bits as the fused tile
+// would at KB = 1 and streams at the HBM rate (the fused tile at KB = 1
+// re-reads its 2-cell tile overlaps and waits at a barrier per plane).
 #include "h3d_tiles.h"
 
 namespace {
@@ -127,10 +134,81 @@ int launch_kb(int ry, int waves, float *uo, const float *ui, const float *kf,
   return H3D_EARG;
 }
 
+// Single-step passes (KB = 1): wave-independent tiles of 64 x (waves * RY)
+// cells marching z, one (tile, z chunk) per workgroup, XCD-aware order.
+template <int RY, int MODE, bool BF16>
+__global__ void __launch_bounds__(512)
+h3d_march_f32(float *__restrict__ uout, const float *__restrict__ uin,
+              const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
+              unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
+              int nx, int ny, int nz, int sy, long long sz, float factor,
+              int zchunk, int tiles_x, int tiles_y) {
+  const unsigned b = h3d_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = (int)(b % (unsigned)tiles_x);
+  const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
+  const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
+  march_f32_tile<RY, MODE, BF16>(uout, uin, kf, kb16, cout, cin, nx, ny, nz, sy, sz,
+                                 factor, zchunk, tx, ty, tz);
+}
+
+template <int RY, int MODE, bool BF16>
+int launch_march_one(int waves, float *uo, const float *ui, const float *kf,
+                     const uint16_t *kb, uint16_t *co, const uint16_t *ci,
+                     const h3d_grid *g, float factor, int zchunk, hipStream_t s) {
+  const int tiles_x = (g->nx - 2 + 63) / 64;
+  const int tiles_y = (g->ny - 2 + RY * waves - 1) / (RY * waves);
+  const long long nb =
+      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+  if (nb > 0x7fffffffLL) return H3D_ELIMIT;
+  hipLaunchKernelGGL((h3d_march_f32<RY, MODE, BF16>), dim3((unsigned)nb), dim3(64 * waves),
+                     0, s, uo, ui, kf, (const unsigned short *)kb, (unsigned short *)co,
+                     (const unsigned short *)ci, g->nx, g->ny, g->nz, g->sy, g->sz, factor,
+                     zchunk, tiles_x, tiles_y);
+  return h3d_launch_status();
+}
+
+template <int RY>
+int launch_march_ry(int waves, float *uo, const float *ui, const float *kf,
+                    const uint16_t *kb, uint16_t *co, const uint16_t *ci,
+                    const h3d_grid *g, float factor, int mode, int zchunk,
+                    hipStream_t s) {
+  const bool bf = kb != nullptr;
+#define H3D_M(M) \
+  return bf ? launch_march_one<RY, M, true>(waves, uo, ui, kf, kb, co, ci, g, factor, zchunk, s) \
+            : launch_march_one<RY, M, false>(waves, uo, ui, kf, kb, co, ci, g, factor, zchunk, s)
+  if (mode == 0) H3D_M(0);
+  if (mode == 1) H3D_M(1);
+  H3D_M(2);
+#undef H3D_M
+}
+
+int launch_march(int ry, int waves, float *uo, const float *ui, const float *kf,
+                 const uint16_t *kb, uint16_t *co, const uint16_t *ci,
+                 const h3d_grid *g, float factor, int mode, int zchunk, hipStream_t s) {
+  if (waves < 1 || waves > 8) return H3D_EARG;
+  if (ry == 1) return launch_march_ry<1>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
+  if (ry == 2) return launch_march_ry<2>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
+  if (ry == 4) return launch_march_ry<4>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
+  return H3D_EARG;
+}
+
 }  // namespace
 
 int h3d_default_tune_f32(const h3d_grid *g, int kblock, h3d_tune *t) {
   t->kblock = kblock > 0 ? kblock : 2;
+  if (t->kblock == 1) {
+    // single-step tiles: the FP64 step's rule (whole z when that gives
+    // 2048 wavefronts, else shorter chunks)
+    t->rows = 2;
+    t->waves = 4;
+    const int planes = g->nz - 2;
+    const long long tiles = (long long)((g->nx - 2 + 63) / 64) *
+                            ((g->ny - 2 + t->rows * t->waves - 1) / (t->rows * t->waves));
+    int zc = planes;
+    while (zc > 16 && tiles * t->waves * ((planes + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    t->zchunk = zc;
+    return H3D_OK;
+  }
   t->rows = 2;
   t->waves = 8;
   const int OX = 64 - 2 * t->kblock, OY = t->rows * t->waves - 2 * t->kblock;
@@ -164,9 +242,12 @@ extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
   if (t.zchunk < 1) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   switch (nsteps) {
+    case 1:
+      return launch_march(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out,
+                          comp_in, g, factor, mode, t.zchunk, s);
 #define H3D_KB(K) \
     case K: return launch_kb<K>(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor, mode, t.zchunk, s);
-    H3D_KB(1) H3D_KB(2) H3D_KB(3) H3D_KB(4) H3D_KB(5) H3D_KB(6)
+    H3D_KB(2) H3D_KB(3) H3D_KB(4) H3D_KB(5) H3D_KB(6)
 #undef H3D_KB
   }
   return H3D_EARG;

@@ -471,4 +471,153 @@ __device__ __forceinline__ void pass_f32_tile(
   }
 }
 
+// One tile (tx, ty, tz) of a single-step FP32 pass (KB = 1), built like the
+// FP64 tile: waves x 64 lanes, RY rows per lane, independent wavefronts (no
+// LDS, no barrier), z-neighbours in registers, x-neighbours through DPP with
+// lanes 0 and 63 loading the x halo, y halo rows loaded per wave.  All
+// accesses are raw buffer instructions with per-plane resources; an access
+// outside the array or off the tile's interior cells carries an
+// out-of-range offset, so it reads 0 or writes nothing without a branch.
+// The update is the same FP32 expression as the temporally blocked pass at
+// KB = 1 (each face term (k_a + k_b)(u_a - u_b) is evaluated directly, with
+// the operands the pass's shared flux has, so the results are the same
+// bits); MODE 2 reads and writes the FP16 residual field.
+template <int RY, int MODE, bool BF16>
+__device__ __forceinline__ void march_f32_tile(
+    float *__restrict__ uout, const float *__restrict__ uin,
+    const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
+    unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
+    int nx, int ny, int nz, int sy, long long sz, float factor, int zchunk,
+    int tx, int ty, int tz) {
+#pragma clang fp contract(off)
+  constexpr unsigned OOB = 0x80000000u;  // past every plane (<= 2^28 elements)
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  const int i = 1 + tx * 64 + lane;
+  const int j0 = 1 + (ty * waves + wave) * RY;
+  if (j0 > ny - 2) return;  // wave-uniform; no barriers in this function
+  const int kb = 1 + tz * zchunk;
+  if (kb > nz - 2) return;
+  const int ke = min(nz - 2, kb + zchunk - 1);
+  const unsigned b4 = (unsigned)sz * 4u, b2 = (unsigned)sz * 2u;
+
+  const bool lane_ld = i <= nx - 1;
+  const bool lane_cmp = i <= nx - 2;
+  const bool xh_ld = (lane == 0) || (lane == 63 && lane_cmp);
+  const int xh_di = (lane == 0) ? -1 : 1;
+  // element offsets -> byte offsets, OOB where nothing is to be read/written
+  unsigned ro[RY], xo[RY], so[RY];
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int j = j0 + r;
+    const unsigned e = (unsigned)(j * sy + i);
+    ro[r] = (lane_ld && j <= ny - 1) ? e : OOB;
+    xo[r] = (xh_ld && j <= ny - 2) ? e + xh_di : OOB;
+    so[r] = (lane_cmp && j <= ny - 2) ? e : OOB;
+  }
+  const unsigned lo = lane_ld ? (unsigned)((j0 - 1) * sy + i) : OOB;
+  const unsigned hi = (lane_ld && j0 + RY <= ny - 1) ? (unsigned)((j0 + RY) * sy + i) : OOB;
+  auto b4o = [](unsigned e) { return e == OOB ? OOB : e * 4u; };
+  auto b2o = [](unsigned e) { return e == OOB ? OOB : e * 2u; };
+  auto ldu = [&](int p, unsigned e) {
+    return ld_f32(plane_rsrc(uin + (long long)p * sz, b4), b4o(e));
+  };
+  auto ldk = [&](int p, unsigned e) {
+    if (BF16) return bf16_to_f32(ld_u16(plane_rsrc(kb16 + (long long)p * sz, b2), b2o(e)));
+    return ld_f32(plane_rsrc(kf + (long long)p * sz, b4), b4o(e));
+  };
+  auto ldc = [&](int p, unsigned e) {
+    return ld_u16(plane_rsrc(cin + (long long)p * sz, b2), b2o(e));
+  };
+
+  float um[RY], uc[RY], up[RY], km[RY], kc[RY], kq[RY], uxh[RY], kxh[RY];
+  unsigned short cc[RY];
+  float ulo, uhi, klo, khi;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    um[r] = ldu(kb - 1, ro[r]);
+    km[r] = ldk(kb - 1, ro[r]);
+    uc[r] = ldu(kb, ro[r]);
+    kc[r] = ldk(kb, ro[r]);
+    up[r] = ldu(kb + 1, ro[r]);
+    kq[r] = ldk(kb + 1, ro[r]);
+    uxh[r] = ldu(kb, xo[r]);
+    kxh[r] = ldk(kb, xo[r]);
+    if (MODE == 2) cc[r] = ldc(kb, ro[r]);
+  }
+  ulo = ldu(kb, lo);
+  klo = ldk(kb, lo);
+  uhi = ldu(kb, hi);
+  khi = ldk(kb, hi);
+
+  for (int k = kb; k <= ke; ++k) {
+    // Prefetch halos of plane k+1 and rows of plane k+2 (planes clamped
+    // into the array; the values fetched past ke are never used).
+    const int p1 = min(k + 1, nz - 1), p2 = min(k + 2, nz - 1);
+    float nu[RY], nk[RY], nuxh[RY], nkxh[RY], nulo, nuhi, nklo, nkhi;
+    unsigned short nc[RY];
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      nu[r] = ldu(p2, ro[r]);
+      nk[r] = ldk(p2, ro[r]);
+      nuxh[r] = ldu(p1, xo[r]);
+      nkxh[r] = ldk(p1, xo[r]);
+      if (MODE == 2) nc[r] = ldc(p1, ro[r]);
+    }
+    nulo = ldu(p1, lo);
+    nklo = ldk(p1, lo);
+    nuhi = ldu(p1, hi);
+    nkhi = ldk(p1, hi);
+
+    const auto ro4 = plane_rsrc(uout + (long long)k * sz, b4);
+    const auto rc2 = plane_rsrc(MODE == 2 ? cout + (long long)k * sz : nullptr, b2);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      const float u = uc[r], kk = kc[r];
+      const float uim = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(uxh[r]), __float_as_int(u), 0x138, 0xf, 0xf, false));
+      const float uip = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(uxh[r]), __float_as_int(u), 0x130, 0xf, 0xf, false));
+      const float kim = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(kxh[r]), __float_as_int(kk), 0x138, 0xf, 0xf, false));
+      const float kip = __int_as_float(__builtin_amdgcn_update_dpp(
+          __float_as_int(kxh[r]), __float_as_int(kk), 0x130, 0xf, 0xf, false));
+      const float ujm = (r == 0) ? ulo : uc[r - 1];
+      const float ujp = (r == RY - 1) ? uhi : uc[r + 1];
+      const float kjm = (r == 0) ? klo : kc[r - 1];
+      const float kjp = (r == RY - 1) ? khi : kc[r + 1];
+      const float s = (kip + kk) * (uip - u) - (kk + kim) * (u - uim) +
+                      (kjp + kk) * (ujp - u) - (kk + kjm) * (u - ujm) +
+                      (kq[r] + kk) * (up[r] - u) - (kk + km[r]) * (u - um[r]);
+      const float d = factor * s;
+      if (MODE == 0) {
+        st_f32(ro4, b4o(so[r]), u + d);
+      } else {
+        const float e = (MODE == 2) ? comp_decode(cc[r], u) : 0.0f;
+        const float y = d + e;
+        const float t = u + y;
+        st_f32(ro4, b4o(so[r]), t);
+        if (MODE == 2) st_u16(rc2, b2o(so[r]), comp_encode(y - (t - u), t));
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      um[r] = uc[r];
+      km[r] = kc[r];
+      uc[r] = up[r];
+      kc[r] = kq[r];
+      up[r] = nu[r];
+      kq[r] = nk[r];
+      uxh[r] = nuxh[r];
+      kxh[r] = nkxh[r];
+      if (MODE == 2) cc[r] = nc[r];
+    }
+    ulo = nulo;
+    uhi = nuhi;
+    klo = nklo;
+    khi = nkhi;
+  }
+}
+
 }  // namespace

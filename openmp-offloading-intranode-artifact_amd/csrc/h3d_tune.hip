@@ -112,11 +112,19 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   } else if (rc == H3D_OK) {
     const float *kf = kappa_bf16 ? nullptr : (const float *)kappa;
     const uint16_t *kb = kappa_bf16 ? (const uint16_t *)kappa : nullptr;
+    const int planes = g->nz - 2;
     for (int kbk = 1; kbk <= 4 && rc == H3D_OK; ++kbk) {
       if (fixed_kblock && kbk != fixed_kblock) continue;
-      for (int rows : {2, 4})
-        for (int waves : {4, 8})
-          for (int zc : {32, 48, 64, 96, 128}) {
+      // single-step passes use the wave-independent tiles (rows 1-4, any
+      // wave count, long z marches); fused passes the LDS-shared tiles
+      const std::vector<int> rws = kbk == 1 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4};
+      const std::vector<int> wvs = kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8};
+      const std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
+                                            : std::vector<int>{32, 48, 64, 96, 128};
+      for (int rows : rws)
+        for (int waves : wvs)
+          for (int zc : zcs) {
+            if (zc > planes && zc != zcs.front()) continue;
             h3d_tune t{rows, waves, zc, kbk};
             float ms;
             int r = time_best(s, [&] {
