@@ -152,10 +152,13 @@ def leg_cfg3(bw, stream):
     lo, hi = h3d.DeviceArray(n * n * 8), h3d.DeviceArray(n * n * 8)
     ms, fb = h3d.C.c_float(), h3d.C.c_int()
     args = (da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor, lo.ptr, hi.ptr)
-    h3d.call("h3d_run_slab_f64", *args, 10, None, 1, stream.ptr, h3d.C.byref(ms),
+    t = h3d.Tune()
+    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0, stream.ptr,
+             h3d.C.byref(t))
+    h3d.call("h3d_run_slab_f64", *args, 10, h3d.C.byref(t), 1, stream.ptr, h3d.C.byref(ms),
              h3d.C.byref(fb))
-    h3d.call("h3d_run_slab_f64", *args, steps, None, 1, stream.ptr, h3d.C.byref(ms),
-             h3d.C.byref(fb))
+    h3d.call("h3d_run_slab_f64", *args, steps, h3d.C.byref(t), 1, stream.ptr,
+             h3d.C.byref(ms), h3d.C.byref(fb))
     lups = (n - 2) * (n - 2) * (planes - 2) * steps
     # per step: 2 packs + 2 unpacks, each reading and writing one plane
     face_bytes = 4 * 2 * n * n * 8 * steps
@@ -167,6 +170,7 @@ def leg_cfg3(bw, stream):
             "dtype": "f64", "steps": steps, "ms_per_step": round(ms.value / steps, 5),
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
             "slab": [n, n, planes], "nz_split": nzd, "face_bytes": n * n * 8,
+            "tune": [t.rows, t.waves, t.zchunk],
             "nonfinite": float(st[3])}
 
 
@@ -193,8 +197,12 @@ def leg_cfg4(bw, stream):
             f.upload(x)
             fs.append(f)
         ms, fb = h3d.C.c_float(), h3d.C.c_int()
+        t64 = h3d.Tune()
+        h3d.call("h3d_autotune", 0, fs[0].ptr, fs[1].ptr, fs[2].ptr, 0, h3d.C.byref(g), 0, 0,
+                 stream.ptr, h3d.C.byref(t64))
         h3d.call("h3d_run_f64", fs[0].ptr, fs[1].ptr, None, fs[2].ptr, h3d.C.byref(g),
-                 factor, steps, None, 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+                 factor, steps, h3d.C.byref(t64), 0, stream.ptr, h3d.C.byref(ms),
+                 h3d.C.byref(fb))
         keep = fs[1] if fb.value else fs[0]
         for f in fs:
             if f is not keep:
