@@ -221,10 +221,16 @@ typedef struct h3d_member {
 typedef struct h3d_ensemble h3d_ensemble;
 
 /* Builds the tile work list on `device_id` and allocates the FP32
- * members' residual fields (zeroed).  tune is unused (tile shapes are
- * fixed: 256-thread workgroups, 64-plane z chunks). */
+ * members' residual fields (zeroed).  Tiles are fixed in x and y (256-thread
+ * workgroups); tune->zchunk, if non-zero, is the z-march depth (planes per
+ * work item; default 64).  tune may be NULL. */
 int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
                         int count, const h3d_tune *tune, int device_id);
+/* Times the ensemble's step for z-march depths 32, 64, 128 and 256 planes
+ * and keeps the fastest (written to out->zchunk if out != NULL).  The timed
+ * launches recompute the next step into the idle buffers, so the members'
+ * state and the results of later runs are unchanged. */
+int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *out);
 /* `steps` time steps of every member, one launch per step.  graph != 0
  * captures the `steps` launches into one hipGraph (kept and reused while
  * the step count and starting parity repeat) and launches that.  *ms is

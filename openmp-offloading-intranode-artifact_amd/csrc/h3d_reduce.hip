@@ -3,8 +3,9 @@
 //
 // Every reduction is two deterministic passes: a fixed grid of workgroups
 // writes one partial record each (per-lane Neumaier-compensated sums,
-// then a fixed-order LDS tree), and a single workgroup folds the partials
-// in index order.  The grid depends only on the array shape, so repeated
+// then a lane-shuffle butterfly inside each wavefront and the wavefront
+// results in order), and a single workgroup folds the partials in index
+// order.  The grid depends only on the array shape, so repeated
 // calls give bit-identical results.  Inputs are read once: these kernels
 // are HBM-bound like the stencil.
 #include <math.h>
@@ -102,19 +103,41 @@ struct Fmax {
   __device__ void merge(const Fmax &o) { m = fmax(m, o.m); }
 };
 
+// A record moved between lanes one dword at a time (records are plain
+// doubles).
+template <typename R>
+__device__ R shfl_xor_record(const R &r, int mask) {
+  static_assert(sizeof(R) % sizeof(int) == 0, "record of dwords");
+  R o;
+  const int *src = reinterpret_cast<const int *>(&r);
+  int *dst = reinterpret_cast<int *>(&o);
+#pragma unroll
+  for (int k = 0; k < (int)(sizeof(R) / sizeof(int)); ++k)
+    dst[k] = __shfl_xor(src[k], mask, 64);
+  return o;
+}
+
+// Workgroup fold: a butterfly over the 64 lanes of each wavefront (lane
+// shuffles, fixed pairing, so the result is deterministic), then the RB/64
+// wavefront results merged in wavefront order.  The result is in sh[0].
 template <typename R>
 __device__ void block_fold(R &acc, R *sh) {
-  sh[threadIdx.x] = acc;
-  __syncthreads();
-  for (int w = RB / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) sh[threadIdx.x].merge(sh[threadIdx.x + w]);
-    __syncthreads();
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) {
+    const R o = shfl_xor_record(acc, m);
+    acc.merge(o);
   }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) sh[wave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int w = 1; w < RB / 64; ++w) sh[0].merge(sh[w]);
+  __syncthreads();
 }
 
 template <typename R>
 __global__ void __launch_bounds__(RB) fold_partials(R *part, int n) {
-  __shared__ R sh[RB];
+  __shared__ R sh[RB / 64];
   R acc;
   acc.init();
   // lane t folds partials t, t+RB, ... in order, then a fixed tree
@@ -129,7 +152,7 @@ template <typename T>
 __global__ void __launch_bounds__(RB)
 stats_kernel(const T *__restrict__ u, int nx, int ny, int nz, int sy,
              long long sz, Stats *part) {
-  __shared__ Stats sh[RB];
+  __shared__ Stats sh[RB / 64];
   Stats acc;
   acc.init();
   const long long rows = (long long)ny * nz;
@@ -147,7 +170,7 @@ __global__ void __launch_bounds__(RB)
 cmp_kernel(const double *__restrict__ ref, const T *__restrict__ cand, int nx,
            int ny, int nz, int sy, long long sz, double atol, double rtol,
            Cmp *part) {
-  __shared__ Cmp sh[RB];
+  __shared__ Cmp sh[RB / 64];
   Cmp acc;
   acc.init();
   const long long rows = (long long)ny * nz;
@@ -165,7 +188,7 @@ cmp_kernel(const double *__restrict__ ref, const T *__restrict__ cand, int nx,
 __global__ void __launch_bounds__(RB)
 fmax_kernel(const double *__restrict__ kp, int nx, int ny, int nz, int sy,
             long long sz, Fmax *part) {
-  __shared__ Fmax sh[RB];
+  __shared__ Fmax sh[RB / 64];
   Fmax acc;
   acc.init();
   const long long rows = (long long)(ny - 2) * (nz - 2);

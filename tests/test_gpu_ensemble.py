@@ -73,6 +73,44 @@ def test_members_equal_standalone_runs(gpu, oracle, graph):
     gpu.call("h3d_ensemble_destroy", e)
 
 
+@pytest.mark.parametrize("zchunk", [0, 3, 32])
+def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
+    """The z-march depth changes the work list, not the arithmetic: members
+    stay bit-identical to standalone runs for a given depth, and the
+    autotuner's timed launches (between two runs, at an odd step count)
+    leave the members' state untouched."""
+    specs = [make_member(gpu, d, m, 91, p) for m, (d, p) in enumerate(SPECS)]
+    mems, keep = [], []
+    for sp in specs:
+        da, db, dk = (gpu.DeviceArray.from_host(x) for x in (sp["u"], sp["u"], sp["k"]))
+        keep.append((da, db, dk))
+        mems.append(gpu.Member(da.ptr, db.ptr, dk.ptr, sp["g"], sp["factor"], sp["precision"]))
+    arr = (gpu.Member * len(mems))(*mems)
+    e = C.c_void_p()
+    t = gpu.Tune(0, 0, zchunk, 0)
+    gpu.call("h3d_ensemble_create", C.byref(e), arr, len(mems), C.byref(t), 0)
+    s = gpu.Stream()
+    run(gpu, e, 3, 1, s)
+    out = gpu.Tune()
+    gpu.call("h3d_ensemble_autotune", e, s.ptr, C.byref(out))
+    assert out.zchunk in (32, 64, 128, 256)
+    run(gpu, e, 2, 1, s)
+    steps = 5
+    for sp, (da, db, dk) in zip(specs, keep):
+        got = db.download()
+        g = sp["g"]
+        if sp["precision"] == 0:
+            want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
+                                  g.nz, steps, factor=sp["factor"])
+            assert np.array_equal(got, want), g.nx
+        else:
+            f32 = float(np.float32(sp["factor"]))
+            want, _ = oracle.run_f32(sp["u"], sp["k"], None, g.nx, g.ny, g.nz, f32, steps,
+                                     1, 2)
+            assert got.tobytes() == want.tobytes(), g.nx
+    gpu.call("h3d_ensemble_destroy", e)
+
+
 def test_graph_equals_eager_over_many_steps(gpu):
     specs = [make_member(gpu, d, m, 5, p) for m, (d, p) in enumerate(SPECS[1:5])]
     outs = []
