@@ -9,13 +9,32 @@
 
 namespace {
 
+// Raw buffer access to one plane: the resource (base = the plane's first
+// element, range = its bytes) is built in scalar registers and each lane
+// supplies a 32-bit byte offset, so a load or store costs no 64-bit vector
+// address arithmetic.  An offset at or past the range reads 0 and drops a
+// write, which is how the tiles below predicate edge lanes and rows.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base,
+                                                            unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes,
+                                           0x00020000);
+}
+__device__ __forceinline__ double ld_f64(__amdgpu_buffer_rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void st_f64(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+  using v2u = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2u, v), r, (int)off, 0, 0);
+}
+
 // One tile (tx, ty, tz) of the FP64 step: waves x 64 lanes, RY rows per
 // lane, planes k_start + tz*zchunk ... (see h3d_f64.hip).  Wavefronts are
 // independent (no barrier), so a wave with nothing to do returns at once.
-// Loads outside the array are predicated off rather than clamped, and the
-// x halo is loaded by lanes 0 and 63 only: on MI355X both measured faster
-// (by 4% and 7% at 512^3) than unconditional clamped loads, and plain
-// stores beat nontemporal ones by 4%.
+// z-neighbours stay in registers, x-neighbours come from the adjacent lanes
+// (DPP) with lanes 0 and 63 loading the x halo, and each wave loads its y
+// halo rows.  Accesses are buffer instructions; lanes and rows outside the
+// array (loads) or off the interior (stores) carry an out-of-range offset
+// instead of a branch.
 template <int RY, bool RHS>
 __device__ __forceinline__ void march_f64_tile(
     double *__restrict__ un, const double *__restrict__ uo,
@@ -23,6 +42,8 @@ __device__ __forceinline__ void march_f64_tile(
     int ny, int sy, long long sz, double factor, int k_start, int k_end,
     int zchunk, int tx, int ty, int tz) {
 #pragma clang fp contract(off)
+  // past every plane: planes hold at most 2^28 elements (2^31 bytes)
+  constexpr unsigned OOB = 0xfffffff8u;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
@@ -33,6 +54,8 @@ __device__ __forceinline__ void march_f64_tile(
   const int kb = k_start + tz * zchunk;
   if (kb > k_end) return;
   const int ke = min(k_end, kb + zchunk - 1);
+  const int nz_last = k_end + 1;  // last plane any load of this range needs
+  const unsigned b8 = (unsigned)((unsigned long long)sz * 8ull);
 
   const bool lane_ld = i <= nx - 1;   // this lane's column exists
   const bool lane_cmp = i <= nx - 2;  // and is interior
@@ -41,75 +64,68 @@ __device__ __forceinline__ void march_f64_tile(
   const bool xh_ld = (lane == 0) || (lane == 63 && lane_cmp);
   const int xh_di = (lane == 0) ? -1 : 1;
 
-  int off[RY];
-  bool row_ld[RY], row_cmp[RY];
+  unsigned ro[RY], xo[RY], so[RY];
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r;
-    off[r] = j * sy + i;
-    row_ld[r] = j <= ny - 1;
-    row_cmp[r] = j <= ny - 2;
+    const unsigned e = (unsigned)(j * sy + i);
+    ro[r] = (lane_ld && j <= ny - 1) ? e * 8u : OOB;
+    xo[r] = (xh_ld && j <= ny - 2) ? (e + xh_di) * 8u : OOB;
+    so[r] = (lane_cmp && j <= ny - 2) ? e * 8u : OOB;
   }
-  const int lo_off = (j0 - 1) * sy + i;
-  const int hi_off = (j0 + RY) * sy + i;
-  const bool hi_ld = j0 + RY <= ny - 1;
-
-  auto ld = [&](const double *p, int o, bool pred) -> double {
-    return pred ? p[o] : 0.0;
-  };
+  const unsigned lo = lane_ld ? (unsigned)((j0 - 1) * sy + i) * 8u : OOB;
+  const unsigned hi =
+      (lane_ld && j0 + RY <= ny - 1) ? (unsigned)((j0 + RY) * sy + i) * 8u : OOB;
+  auto rs = [&](const double *p, int plane) { return plane_rsrc(p + (long long)plane * sz, b8); };
 
   double um[RY], uc[RY], up[RY], km[RY], kc[RY], kq[RY];
   double uxh[RY], kxh[RY], ulo, uhi, klo, khi;
   double rc[RY];
-
   {
-    const double *u0 = uo + (long long)(kb - 1) * sz;
-    const double *k0 = kap + (long long)(kb - 1) * sz;
-    const double *u1 = u0 + sz, *k1 = k0 + sz;
-    const double *u2 = u1 + sz, *k2 = k1 + sz;
+    const auto u0 = rs(uo, kb - 1), k0 = rs(kap, kb - 1);
+    const auto u1 = rs(uo, kb), k1 = rs(kap, kb);
+    const auto u2 = rs(uo, kb + 1), k2 = rs(kap, kb + 1);
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      const bool p = lane_ld && row_ld[r];
-      um[r] = ld(u0, off[r], p);
-      km[r] = ld(k0, off[r], p);
-      uc[r] = ld(u1, off[r], p);
-      kc[r] = ld(k1, off[r], p);
-      up[r] = ld(u2, off[r], p);
-      kq[r] = ld(k2, off[r], p);
-      uxh[r] = ld(u1, off[r] + xh_di, xh_ld && row_cmp[r]);
-      kxh[r] = ld(k1, off[r] + xh_di, xh_ld && row_cmp[r]);
-      if (RHS) rc[r] = ld(rhs + (long long)kb * sz, off[r], p);
+      um[r] = ld_f64(u0, ro[r]);
+      km[r] = ld_f64(k0, ro[r]);
+      uc[r] = ld_f64(u1, ro[r]);
+      kc[r] = ld_f64(k1, ro[r]);
+      up[r] = ld_f64(u2, ro[r]);
+      kq[r] = ld_f64(k2, ro[r]);
+      uxh[r] = ld_f64(u1, xo[r]);
+      kxh[r] = ld_f64(k1, xo[r]);
+      if (RHS) rc[r] = ld_f64(rs(rhs, kb), ro[r]);
     }
-    ulo = ld(u1, lo_off, lane_ld);
-    klo = ld(k1, lo_off, lane_ld);
-    uhi = ld(u1, hi_off, lane_ld && hi_ld);
-    khi = ld(k1, hi_off, lane_ld && hi_ld);
+    ulo = ld_f64(u1, lo);
+    klo = ld_f64(k1, lo);
+    uhi = ld_f64(u1, hi);
+    khi = ld_f64(k1, hi);
   }
 
   for (int k = kb; k <= ke; ++k) {
-    // Prefetch: halos of plane k+1 and rows of plane k+2.
+    // Prefetch: halos of plane k+1 and rows of plane k+2 (plane indices
+    // clamped to the range's last plane; values fetched past ke are unused).
+    const int p1 = min(k + 1, nz_last), p2 = min(k + 2, nz_last);
     double nu[RY], nk[RY], nuxh[RY], nkxh[RY], nulo, nuhi, nklo, nkhi, nr[RY];
-    const bool more = k < ke;
     {
-      const double *u1 = uo + (long long)(k + 1) * sz;
-      const double *k1 = kap + (long long)(k + 1) * sz;
-      const double *u2 = u1 + sz, *k2 = k1 + sz;
+      const auto u1 = rs(uo, p1), k1 = rs(kap, p1);
+      const auto u2 = rs(uo, p2), k2 = rs(kap, p2);
 #pragma unroll
       for (int r = 0; r < RY; ++r) {
-        const bool p = more && lane_ld && row_ld[r];
-        nu[r] = ld(u2, off[r], p);
-        nk[r] = ld(k2, off[r], p);
-        nuxh[r] = ld(u1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
-        nkxh[r] = ld(k1, off[r] + xh_di, more && xh_ld && row_cmp[r]);
-        if (RHS) nr[r] = ld(rhs + (long long)(k + 1) * sz, off[r], p);
+        nu[r] = ld_f64(u2, ro[r]);
+        nk[r] = ld_f64(k2, ro[r]);
+        nuxh[r] = ld_f64(u1, xo[r]);
+        nkxh[r] = ld_f64(k1, xo[r]);
+        if (RHS) nr[r] = ld_f64(rs(rhs, p1), ro[r]);
       }
-      nulo = ld(u1, lo_off, more && lane_ld);
-      nklo = ld(k1, lo_off, more && lane_ld);
-      nuhi = ld(u1, hi_off, more && lane_ld && hi_ld);
-      nkhi = ld(k1, hi_off, more && lane_ld && hi_ld);
+      nulo = ld_f64(u1, lo);
+      nklo = ld_f64(k1, lo);
+      nuhi = ld_f64(u1, hi);
+      nkhi = ld_f64(k1, hi);
     }
 
-    double *out = un + (long long)k * sz;
+    const auto out = rs(un, k);
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       const double u = uc[r], kk = kc[r];
@@ -127,7 +143,7 @@ __device__ __forceinline__ void march_f64_tile(
       double res;
       if (RHS) res = (u + rc[r]) + factor * s;
       else res = u + factor * s;
-      if (lane_cmp && row_cmp[r]) out[off[r]] = res;
+      st_f64(out, so[r], res);
     }
 
 #pragma unroll
@@ -184,17 +200,6 @@ __device__ __forceinline__ float lane_left(float v) {  // lane l <- l-1
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
-// Plane access through raw buffer instructions: one resource per plane
-// (base = the plane's first element, range = its bytes, built in scalar
-// registers) and a 32-bit per-lane byte offset, so a load or store costs
-// no 64-bit vector address arithmetic.  Offsets are always inside the
-// plane (loads are clamped, stores predicated); the range check of the
-// resource would turn a stray access into a zero read / dropped write.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base,
-                                                            unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), 0, (int)bytes,
-                                           0x00020000);
-}
 __device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
 }
