@@ -67,6 +67,8 @@ CASES = [
     ((5, 5, 5), 2, 2, 4, 1),
     ((70, 40, 30), 2, 4, 8, 29),
     ((9, 75, 12), 1, 4, 4, 11),
+    ((70, 40, 30), 1, 1, 2, 0),
+    ((130, 35, 19), 1, 4, 8, 7),
 ]
 
 
