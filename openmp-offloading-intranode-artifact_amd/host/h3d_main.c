@@ -25,6 +25,9 @@
  *   --kblock K         FP32: time steps fused per HBM pass (1..6)
  *   --tile R,W,Z       rows per lane, waves per workgroup, planes per chunk
  *   --autotune         time candidate launch shapes first and use the best
+ *                      (the default when there are at least 128^3 interior
+ *                      points and no --tile is given)
+ *   --no-autotune      use the library's default launch shapes
  *   --graph            replay the step loop from a hipGraph
  *   --slab P:p         run slab p of a P-way z split (P = 2 or 4) with its
  *                      two ghost planes refreshed every step by in-HBM face
@@ -88,7 +91,7 @@ static void usage(void)
   fprintf(stderr,
           "usage: h3d [n [T]] [--steps S] [--mode reference|seeded] "
           "[--kappa sine|lognormal] [--seed N] [--precision f64|f32|f32c|f32c16] "
-          "[--kappa-bf16] [--kblock K] [--tile R,W,Z] [--autotune] [--graph] "
+          "[--kappa-bf16] [--kblock K] [--tile R,W,Z] [--autotune|--no-autotune] [--graph] "
           "[--slab P:p] [--check] [--device D]\n");
   exit(1);
 }
@@ -113,6 +116,7 @@ static opts parse(int argc, char **argv)
   o.T = 1.0;
   o.steps = -1;
   o.seed = 20260101ULL;
+  o.autotune = -1; /* decided from the problem size */
   int pos = 0;
   for (int a = 1; a < argc; a++) {
     const char *s = argv[a];
@@ -143,6 +147,7 @@ static opts parse(int argc, char **argv)
         if (sscanf(v, "%d,%d,%d", &o.tune.rows, &o.tune.waves, &o.tune.zchunk) < 2) usage();
         a++;
       } else if (!strcmp(s, "--autotune")) o.autotune = 1;
+      else if (!strcmp(s, "--no-autotune")) o.autotune = 0;
       else if (!strcmp(s, "--graph")) o.graph = 1;
       else if (!strcmp(s, "--check")) o.check = 1;
       else if (!strcmp(s, "--slab") && v) {
@@ -330,6 +335,12 @@ int main(int argc, char **argv)
 
   h3d_tune tune = o.tune;
   if (f32 && tune.kblock == 0) tune.kblock = 2;
+  if (o.autotune < 0) {
+    /* tuning costs tens of launches: worth it for long runs on large grids,
+     * and the shapes give bit-identical results, so only speed changes */
+    const double interior = (double)(g.nx - 2) * (g.ny - 2) * (g.nz - 2);
+    o.autotune = interior >= 128.0 * 128.0 * 128.0 && o.tune.rows == 0 && o.tune.waves == 0;
+  }
   if (o.autotune) {
     h3d_tune t;
     CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &g, mode,

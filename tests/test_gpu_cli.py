@@ -140,3 +140,15 @@ def test_verify_metrics_and_failures_match_oracle(oracle, tmp_path):
     # nothing to compare -> 2
     r = verify(["--ref", "ref.txt", "missing.txt"], str(tmp_path))
     assert r.returncode == 2 and "no candidate GPU output files found" in r.stdout
+
+
+def test_cli_autotunes_large_grids_with_identical_results(tmp_path):
+    """From 128^3 interior points the program tunes its launch shapes by
+    default; every shape computes the same bits, so the checksum line is
+    the same as with --no-autotune."""
+    args = ["130", "--steps", "20", "--mode", "seeded"]
+    tuned = run(args, str(tmp_path), write=False)
+    plain = run(args + ["--no-autotune"], str(tmp_path), write=False)
+    assert "Autotuned:" in tuned and "Autotuned:" not in plain
+    line = lambda out: next(ln for ln in out.splitlines() if ln.startswith("checksum ="))
+    assert line(tuned) == line(plain)
