@@ -263,28 +263,26 @@ extern "C" int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
   if (graph && !s) return H3D_EARG;
   const int kb = (tune && tune->kblock > 0) ? tune->kblock : 2;
   if (kb > 6) return H3D_ELIMIT;
-  hipEvent_t e0, e1;
-  H3D_TRY(hipEventCreate(&e0));
-  H3D_TRY(hipEventCreate(&e1));
+  h3d_run_res R;
+  H3D_TRY(hipEventCreate(&R.e0));
+  H3D_TRY(hipEventCreate(&R.e1));
   const int full = steps / kb, rem = steps % kb;
   int rc = H3D_OK;
-  hipGraph_t gr = nullptr;
-  hipGraphExec_t ge = nullptr;
   if (graph && full >= 2) {
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int r1 = h3d_pass_f32(u_b, u_a, kappa_f32, kappa_bf16, c_b, c_a, g, factor, kb, mode, tune, s);
     int r2 = h3d_pass_f32(u_a, u_b, kappa_f32, kappa_bf16, c_a, c_b, g, factor, kb, mode, tune, s);
-    hipError_t ec = hipStreamEndCapture(s, &gr);
+    hipError_t ec = hipStreamEndCapture(s, &R.gr);
     if (r1 != H3D_OK) return r1;
     if (r2 != H3D_OK) return r2;
     H3D_TRY(ec);
-    H3D_TRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    H3D_TRY(hipGraphInstantiate(&R.ge, R.gr, nullptr, nullptr, 0));
   }
-  H3D_TRY(hipEventRecord(e0, s));
+  H3D_TRY(hipEventRecord(R.e0, s));
   int pass = 0;
-  if (ge) {
+  if (R.ge) {
     for (; pass + 2 <= full; pass += 2) {
-      hipError_t el = hipGraphLaunch(ge, s);
+      hipError_t el = hipGraphLaunch(R.ge, s);
       if (el != hipSuccess) { rc = (int)el; break; }
     }
   }
@@ -297,16 +295,12 @@ extern "C" int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
                       tune, s);
   }
   if (rc == H3D_OK) {
-    H3D_TRY(hipEventRecord(e1, s));
-    H3D_TRY(hipEventSynchronize(e1));
+    H3D_TRY(hipEventRecord(R.e1, s));
+    H3D_TRY(hipEventSynchronize(R.e1));
     float tms = 0.f;
-    H3D_TRY(hipEventElapsedTime(&tms, e0, e1));
+    H3D_TRY(hipEventElapsedTime(&tms, R.e0, R.e1));
     if (ms) *ms = tms;
     if (final_in_b) *final_in_b = npass % 2;
   }
-  if (ge) (void)hipGraphExecDestroy(ge);
-  if (gr) (void)hipGraphDestroy(gr);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }

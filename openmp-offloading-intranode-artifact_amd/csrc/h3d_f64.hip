@@ -184,27 +184,25 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
   if (!u_a || !u_b || !kappa || !h3d_grid_ok(g) || steps < 0) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
-  hipEvent_t e0, e1;
-  H3D_TRY(hipEventCreate(&e0));
-  H3D_TRY(hipEventCreate(&e1));
+  h3d_run_res R;
+  H3D_TRY(hipEventCreate(&R.e0));
+  H3D_TRY(hipEventCreate(&R.e1));
   int rc = H3D_OK;
-  hipGraph_t gr = nullptr;
-  hipGraphExec_t ge = nullptr;
   if (graph && steps >= 2) {
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int r1 = h3d_step_f64(u_b, u_a, rhs, kappa, g, factor, 1, g->nz - 2, tune, s);
     int r2 = h3d_step_f64(u_a, u_b, rhs, kappa, g, factor, 1, g->nz - 2, tune, s);
-    hipError_t ec = hipStreamEndCapture(s, &gr);
+    hipError_t ec = hipStreamEndCapture(s, &R.gr);
     if (r1 != H3D_OK) return r1;
     if (r2 != H3D_OK) return r2;
     H3D_TRY(ec);
-    H3D_TRY(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
+    H3D_TRY(hipGraphInstantiate(&R.ge, R.gr, nullptr, nullptr, 0));
   }
-  H3D_TRY(hipEventRecord(e0, s));
+  H3D_TRY(hipEventRecord(R.e0, s));
   int done = 0;
-  if (ge) {
+  if (R.ge) {
     for (; done + 2 <= steps; done += 2) {
-      hipError_t el = hipGraphLaunch(ge, s);
+      hipError_t el = hipGraphLaunch(R.ge, s);
       if (el != hipSuccess) { rc = (int)el; break; }
     }
   }
@@ -214,16 +212,12 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
                       factor, 1, g->nz - 2, tune, s);
   }
   if (rc == H3D_OK) {
-    H3D_TRY(hipEventRecord(e1, s));
-    H3D_TRY(hipEventSynchronize(e1));
+    H3D_TRY(hipEventRecord(R.e1, s));
+    H3D_TRY(hipEventSynchronize(R.e1));
     float t = 0.f;
-    H3D_TRY(hipEventElapsedTime(&t, e0, e1));
+    H3D_TRY(hipEventElapsedTime(&t, R.e0, R.e1));
     if (ms) *ms = t;
     if (final_in_b) *final_in_b = steps % 2;
   }
-  if (ge) (void)hipGraphExecDestroy(ge);
-  if (gr) (void)hipGraphDestroy(gr);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   return rc;
 }

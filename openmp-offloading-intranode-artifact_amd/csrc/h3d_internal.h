@@ -34,6 +34,23 @@ static inline bool h3d_grid_ok(const h3d_grid *g) {
   return true;
 }
 
+// Timing events and a captured graph of a run, released on every return
+// path (early returns after a failed launch included).
+struct h3d_run_res {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipGraph_t gr = nullptr;
+  hipGraphExec_t ge = nullptr;
+  h3d_run_res() = default;
+  h3d_run_res(const h3d_run_res &) = delete;
+  h3d_run_res &operator=(const h3d_run_res &) = delete;
+  ~h3d_run_res() {
+    if (ge) (void)hipGraphExecDestroy(ge);
+    if (gr) (void)hipGraphDestroy(gr);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  }
+};
+
 static inline size_t h3d_grid_elems(const h3d_grid *g) {
   return (size_t)g->sz * (size_t)g->nz;
 }
