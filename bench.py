@@ -9,15 +9,21 @@ dt = 0.9 x the FP64 positivity bound, FP64 storage and arithmetic.  One
 (default 1000, as in the configuration), after --warmup untimed steps.
 
 Reported beside it, in the same run:
-  - the measured HBM stream bandwidth (copy / triad probe on 4 GiB arrays),
+  - the measured HBM stream bandwidth (copy / triad probe on 4 GiB arrays)
+    and host <-> device copy rates,
   - the FP32-compensated, temporally blocked kernel on the same field and
-    step count, with its GLUP/s, % of stream bandwidth, speed-up over the
-    FP64 kernel and rel-L2 / max error against an FP64 run on the same
+    step count (the autotuner's fastest fused depth, and single-step
+    passes), with GLUP/s, % of stream bandwidth, speed-up over the FP64
+    kernel and rel-L2 / max error against an FP64 run on the same
     FP32-rounded inputs,
+  - the other configurations: the reference's own GPU case (n = 512,
+    reference fields with rhs, 501 steps), one slab of a 4-way split of
+    1024^3 with in-HBM face exchange, 1024^3 FP32 for 2000 steps in four
+    storage variants, and a 64-member mixed ensemble (--no-configs skips),
   - the CPU baseline: the reference's own serial program (built from its
-    source into oracle/_ref) with OpenMP on this host's cores, on the
-    reference's recorded CPU case (n = 256, 101 steps), and the plain C
-    oracle on one core on 128^3 x 100 steps.
+    source into oracle/_ref) with OpenMP on this host's cores and on one
+    thread, on the reference's recorded CPU case (n = 256, 101 steps), and
+    the plain C oracle on one core on 128^3 x 100 steps.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -138,6 +144,46 @@ def seeded_problem(gx, gy, gz, kind, seed, g=None, gk0=0, member=0):
     fmax = h3d.lib().h3d_face_sum_max_host(h3d.ptr(k), h3d.C.byref(g))
     dt = 0.9 * 2.0 * h * h / fmax
     return g, u, k, dt / h / h / 2.0
+
+
+def leg_reference_workload(bw, stream):
+    """The reference's own GPU benchmark case, for comparison with its
+    published numbers: reference-mode fields (sin/cos closed forms, rhs),
+    dense layout, n = 512 points per axis (510^3 interior) and T = 500 dt
+    printed %.7g, which the step rule turns into 501 steps
+    (AMD-MI250X/Source-Results/HIP/summary.csv:2; submit.sh:61-62).  32 B
+    per update (u, rhs, kappa read; u written)."""
+    n = 512
+    h, dt, factor = h3d.params_reference(n)
+    T = float("%.7g" % (500 * dt))
+    steps = h3d.count_steps(T, dt)
+    g = h3d.Grid.dense(n)
+    u, r, k = h3d.init_reference(n)
+    da, db, dr, dk = (h3d.DeviceArray.from_host(x) for x in (u, u, r, k))
+    t = h3d.Tune()
+    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0, stream.ptr,
+             h3d.C.byref(t))
+    ms, fb = h3d.C.c_float(), h3d.C.c_int()
+    h3d.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr, dk.ptr, h3d.C.byref(g), factor, 10,
+             h3d.C.byref(t), 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+    da.upload(u)
+    db.upload(u)
+    h3d.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr, dk.ptr, h3d.C.byref(g), factor, steps,
+             h3d.C.byref(t), 0, stream.ptr, h3d.C.byref(ms), h3d.C.byref(fb))
+    sec = ms.value / 1e3
+    lups = (n - 2) ** 3 * steps
+    glups = lups / sec / 1e9
+    st = h3d.stats(db if fb.value else da, g)
+    for d in (da, db, dr, dk):
+        d.close()
+    mi250x_hip = 13.06  # GLUP/s, 1 GCD, HIP/summary.csv:2 (5.0899 s for 501 steps)
+    return {"value": round(glups, 3), "unit": "GLUP/s", "dtype": "f64", "n": n,
+            "steps": steps, "T": "%.7g" % T, "seconds": round(sec, 5),
+            "achieved_gbs": round(32.0 * lups / sec / 1e9, 1),
+            "frac_of_stream": round(32.0 * lups / sec / 1e9 / bw, 4),
+            "tune": [t.rows, t.waves, t.zchunk], "checksum": float(st[0]),
+            "reference_mi250x_hip_glups": mi250x_hip,
+            "ratio_to_reference_mi250x_hip": round(glups / mi250x_hip, 2)}
 
 
 def leg_cfg3(bw, stream):
@@ -308,9 +354,11 @@ def leg_cfg5(bw, stream):
     e2, keep2 = build(twins)
     h3d.call("h3d_ensemble_run", e2, 1000, 1, stream.ptr, h3d.C.byref(ms))
     worst = 0.0
-    for (da, db, dk), (ta, tb, tk) in zip(keep[1::2], keep2):
-        got = da.download().astype(np.float64)  # 1000 steps: results in u_a
-        ref = ta.download()
+    in_b = h3d.C.c_int()
+    for mi, ((da, db, dk), (ta, tb, tk)) in enumerate(zip(keep[1::2], keep2)):
+        h3d.call("h3d_ensemble_where", e, 2 * mi + 1, h3d.C.byref(in_b))
+        got = (db if in_b.value else da).download().astype(np.float64)
+        ref = ta.download()  # FP64 twins: 1000 steps, results in u_a
         worst = max(worst, float(np.linalg.norm(got - ref) / np.linalg.norm(ref)))
     h3d.call("h3d_ensemble_destroy", e)
     h3d.call("h3d_ensemble_destroy", e2)
@@ -318,11 +366,13 @@ def leg_cfg5(bw, stream):
         for d in trip:
             d.close()
     # FP64 members 24 B per update; FP32 members 4+4+4 B plus 2+2 B residual
+    # per pass of kblock steps
     b64 = sum(24.0 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
-    b32 = sum(16.0 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
+    b32 = sum(16.0 / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
     gbs = (b64 + b32) * 500 / sec / 1e9
     return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
             "cells": int(cells), "steps_timed": 500, "graph": True, "zchunk": et.zchunk,
+            "fp32_kblock": et.kblock,
             "ms_per_step": round(1e3 * sec / 500, 5),
             "work_items": nwork.value, "workgroups": nblk.value,
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
@@ -510,7 +560,8 @@ def main():
             d.close()
     if not a.no_configs:
         result["configs"] = {}
-        for name, fn in (("slab_1024x1024x256_fp64", leg_cfg3),
+        for name, fn in (("reference_workload_n512_501steps", leg_reference_workload),
+                         ("slab_1024x1024x256_fp64", leg_cfg3),
                          ("fp32c_1024cubed_2000steps", leg_cfg4),
                          ("ensemble_64_members_500steps", leg_cfg5)):
             log(f"{name} ...")

@@ -214,31 +214,38 @@ typedef struct h3d_member {
                         rounded to float)                                   */
   int precision;     /* 0 = FP64 step (as h3d_step_f64); 1 = FP32 storage,
                         compensated, with an FP16 residual field kept by the
-                        ensemble across steps (as h3d_pass_f32, one step,
-                        mode H3D_F32_COMP16)                                */
+                        ensemble across passes (as h3d_pass_f32 with mode
+                        H3D_F32_COMP16 and the ensemble's kblock)           */
 } h3d_member;
 
 typedef struct h3d_ensemble h3d_ensemble;
 
-/* Builds the tile work list on `device_id` and allocates the FP32
+/* Builds the tile work lists on `device_id` and allocates the FP32
  * members' residual fields (zeroed).  Tiles are fixed in x and y (256-thread
- * workgroups); tune->zchunk, if non-zero, is the z-march depth (planes per
- * work item; default 64).  tune may be NULL. */
+ * workgroups).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
+ * depth (planes per work item, default 64) and tune->kblock the FP32
+ * temporal-blocking depth (steps fused per pass of an FP32 member, 1..3,
+ * default 1; H3D_ELIMIT above). */
 int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
                         int count, const h3d_tune *tune, int device_id);
-/* Times the ensemble's step for z-march depths 32, 64, 128 and 256 planes
- * and keeps the fastest (written to out->zchunk if out != NULL).  The timed
- * launches recompute the next step into the idle buffers, so the members'
- * state and the results of later runs are unchanged. */
-int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *out);
-/* `steps` time steps of every member, one launch per step.  graph != 0
- * captures the `steps` launches into one hipGraph (kept and reused while
- * the step count and starting parity repeat) and launches that.  *ms is
- * the device time of the steps.  Fields ping-pong u_a -> u_b -> u_a over
- * the ensemble's whole history: after an odd total number of steps every
- * member's result is in u_b. */
+/* `steps` time steps of every member.  With kblock K, each group of K
+ * steps is one launch (FP64 tiles one step, FP32 tiles a K-step pass) and
+ * K-1 launches over the FP64 tiles; steps past the last full group are
+ * single-step launches.  graph != 0 captures the run's launches into one
+ * hipGraph (kept and reused while the step count and starting parities
+ * repeat) and launches that.  *ms is the device time of the steps.  FP64
+ * members ping-pong u_a -> u_b every step, FP32 members every pass:
+ * h3d_ensemble_where says which buffer holds a member's current field. */
 int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph, void *stream,
                      float *ms);
+/* Times the ensemble's step for z-march depths 32..256 planes and, when it
+ * has FP32 members, FP32 pass depths 1..3, and keeps the fastest per step
+ * (written to *out if out != NULL).  The timed launches recompute the next
+ * step into the idle buffers, so the members' state and the results of
+ * later runs are unchanged. */
+int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *out);
+/* *in_b = 1 if member `member`'s current field is in its u_b, else 0. */
+int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b);
 /* Work-list size, grid size and steps taken so far. */
 int h3d_ensemble_info(const h3d_ensemble *e, int *nwork, int *nblocks,
                       long long *steps_done);

@@ -1,20 +1,25 @@
-// Ensembles: many independent heat problems of uneven sizes, advanced one
-// time step per launch by a persistent kernel over a device-side work list.
+// Ensembles: many independent heat problems of uneven sizes, advanced by a
+// persistent kernel over a device-side work list, one launch per step.
 //
 // Each member is cut into tiles (x segments of 64 lanes, y strips, z
-// chunks of `zchunk` planes: the z-march depth, chosen at creation or by
-// h3d_ensemble_autotune).  The work list holds one int4 {member, tx, ty,
-// tz} per tile, largest members first, and a fixed grid of 256-thread
-// workgroups -- as many as are resident on the device at once -- walks it
-// with a grid stride.  FP64 members run the FP64 tile (2 rows per lane, 4
-// waves: 64 x 8 points per z step), bit-identical to h3d_step_f64.  FP32
-// members run the FP32 pass tile with one fused step and the FP16 residual
-// field (mode H3D_F32_COMP16), so the compensation carries across steps;
-// that tile computes the same bits as h3d_pass_f32 with nsteps = 1 (which
-// runs the wave-independent single-step tile: inside this persistent
-// kernel the LDS tile measured 8% faster over the 64-member ensemble).
-// Both kinds share one launch, so a step of the whole ensemble is one
-// kernel, and `steps` launches are captured into one hipGraph.
+// chunks of `zchunk` planes: the z-march depth).  A work list holds one
+// int4 {member, tx, ty, tz} per tile, costliest first, and a fixed grid of
+// 256-thread workgroups -- as many as are resident on the device at once --
+// walks it with a grid stride.  FP64 members run the FP64 tile (2 rows per
+// lane, 4 waves: 64 x 8 points per z step), bit-identical to h3d_step_f64.
+// FP32 members run the FP32 pass tile with the FP16 residual field (mode
+// H3D_F32_COMP16), so the compensation carries across passes; a pass fuses
+// `kblock` steps (the temporal-blocking depth), bit-identical to
+// h3d_pass_f32 with the same nsteps, rows 4, waves 4.  Inside this
+// persistent kernel the LDS tile measured 8% faster than the
+// wave-independent single-step tile, so it is used for kblock = 1 too.
+//
+// With kblock = K > 1 a group of K steps is one launch in which FP64 tiles
+// take one step and FP32 tiles a K-step pass, then K-1 launches over the
+// FP64 tiles only.  Steps beyond the last full group are single-step
+// launches over every tile.  So FP64 members flip their ping-pong buffers
+// every step and FP32 members once per pass (h3d_ensemble_where).  The
+// launches of a run are captured into one hipGraph on request.
 #include <vector>
 #include <algorithm>
 
@@ -26,6 +31,7 @@ constexpr int ENS_WAVES = 4;    // 256-thread workgroups
 constexpr int ENS_F64_ROWS = 2;
 constexpr int ENS_F32_RY = 4;
 constexpr int ENS_ZCHUNK = 64;  // default z-march depth
+constexpr int ENS_MAXKB = 3;    // deepest FP32 pass fused inside the ensemble
 
 struct EMember {
   void *ua, *ub;
@@ -38,27 +44,29 @@ struct EMember {
   int precision;
 };
 
+// parity64 / parity32: which buffer FP64 / FP32 members read this launch.
+template <int KB>
 __global__ void __launch_bounds__(ENS_WAVES * 64)
 ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
-              int nwork, int parity, int zchunk) {
+              int nwork, int parity64, int parity32, int zchunk) {
   extern __shared__ float lds[];
   for (int t = blockIdx.x; t < nwork; t += gridDim.x) {
     const int4 d = work[t];
     const EMember m = mem[d.x];
     if (m.precision == 0) {
-      const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
-      double *out = static_cast<double *>(parity ? m.ua : m.ub);
+      const double *in = static_cast<const double *>(parity64 ? m.ub : m.ua);
+      double *out = static_cast<double *>(parity64 ? m.ua : m.ub);
       march_f64_tile<ENS_F64_ROWS, false>(out, in, nullptr,
                                           static_cast<const double *>(m.kap),
                                           m.nx, m.ny, m.sy, m.sz, m.factor, 1,
                                           m.nz - 2, zchunk, d.y, d.z, d.w);
     } else {
-      const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
-      float *out = static_cast<float *>(parity ? m.ua : m.ub);
-      const unsigned short *cin = parity ? m.cb : m.ca;
-      unsigned short *cout = parity ? m.ca : m.cb;
+      const float *in = static_cast<const float *>(parity32 ? m.ub : m.ua);
+      float *out = static_cast<float *>(parity32 ? m.ua : m.ub);
+      const unsigned short *cin = parity32 ? m.cb : m.ca;
+      unsigned short *cout = parity32 ? m.ca : m.cb;
       const int zb = 1 + d.w * zchunk;
-      pass_f32_tile<1, ENS_F32_RY, ENS_WAVES, 2, false>(
+      pass_f32_tile<KB, ENS_F32_RY, ENS_WAVES, 2, false>(
           lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
           m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
           min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
@@ -68,88 +76,178 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
   }
 }
 
-constexpr size_t ENS_LDS =
-    sizeof(float) * (size_t)pass_lds_floats(1, ENS_F32_RY, ENS_WAVES);
+template <int KB>
+constexpr size_t ens_lds() {
+  return sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
+}
 
 }  // namespace
 
 struct h3d_ensemble {
   int device;
   int count;
-  int nwork;
-  int nblocks;
   int zchunk;
-  int work_cap;  // int4 slots allocated at d_work
-  struct Tiles { int tiles_x, tiles_y, planes, precision; };
-  std::vector<Tiles> tiles;  // per member
-  long long done;  // steps taken so far (parity of the next launch)
+  int kb;           // FP32 steps per pass
+  long long done;   // steps taken so far
+  long long passes; // FP32 passes taken so far
   EMember *d_mem;
+  struct Tiles { int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision; };
+  std::vector<Tiles> tiles;  // per member; [0] FP64 tiling, [d] FP32 at depth d
+  // Work lists in one device buffer: every tile at depth kb, every tile at
+  // depth 1 (single-step launches), FP64 tiles only.
   int4 *d_work;
+  int work_cap;
+  int off_kb, n_kb, off_1, n_1, off_64, n_64;
+  int blocks_kb, blocks_1;  // persistent grid of each kernel instance
   std::vector<unsigned short *> comp;  // owned FP16 residual fields
-  // cached graph of `graph_steps` launches starting at `graph_parity`
+  // cached graph of `graph_steps` steps starting at parities (p64, p32)
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
-  int graph_steps = -1, graph_parity = -1;
+  int graph_steps = -1, graph_p64 = -1, graph_p32 = -1;
 };
+
+static void drop_graph(h3d_ensemble *e) {
+  if (e->graph) (void)hipGraphExecDestroy(e->graph);
+  if (e->graph_def) (void)hipGraphDestroy(e->graph_def);
+  e->graph = nullptr;
+  e->graph_def = nullptr;
+  e->graph_steps = e->graph_p64 = e->graph_p32 = -1;
+}
 
 static void ensemble_free(h3d_ensemble *e) {
   if (!e) return;
   (void)hipSetDevice(e->device);
-  if (e->graph) (void)hipGraphExecDestroy(e->graph);
-  if (e->graph_def) (void)hipGraphDestroy(e->graph_def);
+  drop_graph(e);
   if (e->d_mem) (void)hipFree(e->d_mem);
   if (e->d_work) (void)hipFree(e->d_work);
   for (auto *c : e->comp) (void)hipFree(c);
   delete e;
 }
 
-// (Re)builds the work list for z chunks of `zchunk` planes: one int4
-// {member, tx, ty, tz} per tile, costliest first (stable, so a member's
-// tiles stay together), and sizes the persistent grid.  Drops any cached
-// graph, whose launches carry the old list length.
-static int build_work(h3d_ensemble *e, int zchunk) {
-  if (zchunk < 1) return H3D_EARG;
-  struct Job { long long cost; int4 d; };
-  std::vector<Job> jobs;
-  for (int m = 0; m < e->count; ++m) {
-    const h3d_ensemble::Tiles &t = e->tiles[m];
-    const int chunks = (t.planes + zchunk - 1) / zchunk;
-    for (int tz = 0; tz < chunks; ++tz) {
-      const int planes = std::min(zchunk, t.planes - tz * zchunk);
-      for (int ty = 0; ty < t.tiles_y; ++ty)
-        for (int tx = 0; tx < t.tiles_x; ++tx)
-          jobs.push_back({(long long)planes * (t.precision ? 2 : 3), {m, tx, ty, tz}});
-    }
+static int occupancy_blocks(int device, int kb, int nwork, int *blocks) {
+  int cus = 0, per_cu = 0;
+  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  hipError_t he;
+  switch (kb) {
+    case 1: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)ensemble_step<1>, ENS_WAVES * 64, ens_lds<1>()); break;
+    case 2: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)ensemble_step<2>, ENS_WAVES * 64, ens_lds<2>()); break;
+    default: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, (const void *)ensemble_step<3>, ENS_WAVES * 64, ens_lds<3>()); break;
   }
-  if (jobs.size() > 0x7fffffffu) return H3D_ELIMIT;
-  std::stable_sort(jobs.begin(), jobs.end(),
-                   [](const Job &a, const Job &b) { return a.cost > b.cost; });
-  std::vector<int4> work(jobs.size());
-  for (size_t j = 0; j < jobs.size(); ++j) work[j] = jobs[j].d;
-  hipError_t he = hipSuccess;
-  if ((int)work.size() > e->work_cap) {
+  H3D_TRY(he);
+  *blocks = std::max(1, std::min(std::max(nwork, 1), cus * std::max(per_cu, 1)));
+  return H3D_OK;
+}
+
+// (Re)builds the three work lists for z chunks of `zchunk` planes and FP32
+// passes of `kb` steps, costliest tiles first (stable, so a member's tiles
+// stay together), and sizes the persistent grids.  Drops any cached graph.
+static int build_work(h3d_ensemble *e, int zchunk, int kb) {
+  if (zchunk < 1 || kb < 1 || kb > ENS_MAXKB) return H3D_EARG;
+  struct Job { long long cost; int4 d; };
+  auto list = [&](int depth, bool f32) {
+    std::vector<Job> jobs;
+    for (int m = 0; m < e->count; ++m) {
+      const h3d_ensemble::Tiles &t = e->tiles[m];
+      if (t.precision == 1 && !f32) continue;
+      const int tx_n = t.precision ? t.tiles_x[depth] : t.tiles_x[0];
+      const int ty_n = t.precision ? t.tiles_y[depth] : t.tiles_y[0];
+      const int chunks = (t.planes + zchunk - 1) / zchunk;
+      for (int tz = 0; tz < chunks; ++tz) {
+        const int planes = std::min(zchunk, t.planes - tz * zchunk);
+        const long long cost = (long long)planes * (t.precision ? 2 * depth : 3);
+        for (int ty = 0; ty < ty_n; ++ty)
+          for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
+      }
+    }
+    std::stable_sort(jobs.begin(), jobs.end(),
+                     [](const Job &a, const Job &b) { return a.cost > b.cost; });
+    std::vector<int4> w(jobs.size());
+    for (size_t j = 0; j < jobs.size(); ++j) w[j] = jobs[j].d;
+    return w;
+  };
+  std::vector<int4> wkb = list(kb, true);
+  std::vector<int4> w1 = kb > 1 ? list(1, true) : std::vector<int4>();
+  std::vector<int4> w64 = kb > 1 ? list(1, false) : std::vector<int4>();
+  const size_t total = wkb.size() + w1.size() + w64.size();
+  if (total > 0x7fffffffu) return H3D_ELIMIT;
+  std::vector<int4> all;
+  all.reserve(total);
+  all.insert(all.end(), wkb.begin(), wkb.end());
+  all.insert(all.end(), w1.begin(), w1.end());
+  all.insert(all.end(), w64.begin(), w64.end());
+  if ((int)total > e->work_cap) {
     if (e->d_work) (void)hipFree(e->d_work);
     e->d_work = nullptr;
     e->work_cap = 0;
-    he = hipMalloc(&e->d_work, sizeof(int4) * work.size());
-    if (he == hipSuccess) e->work_cap = (int)work.size();
+    H3D_TRY(hipMalloc(&e->d_work, sizeof(int4) * std::max<size_t>(total, 1)));
+    e->work_cap = (int)total;
   }
-  if (he == hipSuccess)
-    he = hipMemcpy(e->d_work, work.data(), sizeof(int4) * work.size(), hipMemcpyHostToDevice);
-  int cus = 0, per_cu = 0;
-  if (he == hipSuccess)
-    he = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device);
-  if (he == hipSuccess)
-    he = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)ensemble_step,
-                                                      ENS_WAVES * 64, ENS_LDS);
-  if (he != hipSuccess) return (int)he;
-  e->nwork = (int)work.size();
-  e->nblocks = std::max(1, std::min(e->nwork, cus * std::max(per_cu, 1)));
+  if (total)
+    H3D_TRY(hipMemcpy(e->d_work, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
+  e->off_kb = 0;
+  e->n_kb = (int)wkb.size();
+  // with kb = 1 the single-step list is the kb list itself
+  e->off_1 = kb > 1 ? e->n_kb : 0;
+  e->n_1 = kb > 1 ? (int)w1.size() : e->n_kb;
+  e->off_64 = e->n_kb + (int)w1.size();
+  e->n_64 = (int)w64.size();
+  int rc = occupancy_blocks(e->device, kb, e->n_kb, &e->blocks_kb);
+  if (rc == H3D_OK) rc = occupancy_blocks(e->device, 1, e->n_1, &e->blocks_1);
+  if (rc != H3D_OK) return rc;
   e->zchunk = zchunk;
-  if (e->graph) { (void)hipGraphExecDestroy(e->graph); e->graph = nullptr; }
-  if (e->graph_def) { (void)hipGraphDestroy(e->graph_def); e->graph_def = nullptr; }
-  e->graph_steps = e->graph_parity = -1;
+  e->kb = kb;
+  drop_graph(e);
   return H3D_OK;
+}
+
+// One launch over list 0 (every tile, FP32 passes of kb steps), 1 (every
+// tile, single steps) or 2 (FP64 tiles only).
+static int ens_launch(h3d_ensemble *e, int list, int p64, int p32, hipStream_t s) {
+  const int4 *w = e->d_work + (list == 0 ? e->off_kb : list == 1 ? e->off_1 : e->off_64);
+  const int n = list == 0 ? e->n_kb : list == 1 ? e->n_1 : e->n_64;
+  if (n == 0) return H3D_OK;
+  const int depth = list == 0 ? e->kb : 1;
+  const int blocks = list == 0 ? e->blocks_kb : e->blocks_1;
+  switch (depth) {
+    case 1:
+      hipLaunchKernelGGL(ensemble_step<1>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<1>(),
+                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+      break;
+    case 2:
+      hipLaunchKernelGGL(ensemble_step<2>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<2>(),
+                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+      break;
+    default:
+      hipLaunchKernelGGL(ensemble_step<3>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<3>(),
+                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+      break;
+  }
+  return h3d_launch_status();
+}
+
+// FP32 passes in a run of `steps` steps: full groups plus single steps.
+static int ens_passes(const h3d_ensemble *e, int steps) {
+  const int full = e->kb > 1 ? steps / e->kb : 0;
+  return full + (steps - full * e->kb);
+}
+
+// Issues the launches of `steps` steps from parities (p64, p32).
+static int ens_issue(h3d_ensemble *e, int steps, int p64, int p32, hipStream_t s) {
+  int rc = H3D_OK, np = 0, k = 0;
+  if (e->kb > 1) {
+    for (; rc == H3D_OK && k + e->kb <= steps; k += e->kb) {
+      rc = ens_launch(e, 0, (p64 + k) % 2, (p32 + np) % 2, s);
+      ++np;
+      for (int j = 1; rc == H3D_OK && j < e->kb; ++j)
+        rc = ens_launch(e, 2, (p64 + k + j) % 2, (p32 + np) % 2, s);
+    }
+  }
+  for (; rc == H3D_OK && k < steps; ++k, ++np)
+    rc = ens_launch(e, 1, (p64 + k) % 2, (p32 + np) % 2, s);
+  return rc;
 }
 
 extern "C" int h3d_ensemble_create(h3d_ensemble **out,
@@ -157,6 +255,8 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
                                    const h3d_tune *tune, int device_id) {
   if (!out || !members || count < 1) return H3D_EARG;
   *out = nullptr;
+  const int kb = (tune && tune->kblock > 0) ? tune->kblock : 1;
+  if (kb > ENS_MAXKB) return H3D_ELIMIT;
   int ndev = 0;
   H3D_TRY(hipGetDeviceCount(&ndev));
   if (device_id < 0 || device_id >= ndev) return H3D_ENODEV;
@@ -185,13 +285,14 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
     h3d_ensemble::Tiles t{};
     t.planes = x.g.nz - 2;
     t.precision = x.precision;
-    if (x.precision == 0) {
-      t.tiles_x = (x.g.nx - 2 + 63) / 64;
-      t.tiles_y = (x.g.ny - 2 + ENS_F64_ROWS * ENS_WAVES - 1) / (ENS_F64_ROWS * ENS_WAVES);
-    } else {
-      const int OX = 62, OY = ENS_F32_RY * ENS_WAVES - 2;
-      t.tiles_x = (x.g.nx - 2 + OX - 1) / OX;
-      t.tiles_y = (x.g.ny - 2 + OY - 1) / OY;
+    t.tiles_x[0] = (x.g.nx - 2 + 63) / 64;
+    t.tiles_y[0] = (x.g.ny - 2 + ENS_F64_ROWS * ENS_WAVES - 1) / (ENS_F64_ROWS * ENS_WAVES);
+    for (int d = 1; d <= ENS_MAXKB; ++d) {
+      const int OX = 64 - 2 * d, OY = ENS_F32_RY * ENS_WAVES - 2 * d;
+      t.tiles_x[d] = (x.g.nx - 2 + OX - 1) / OX;
+      t.tiles_y[d] = (x.g.ny - 2 + OY - 1) / OY;
+    }
+    if (x.precision == 1) {
       // residual fields start at zero; they live as long as the ensemble
       const size_t bytes = h3d_grid_elems(&x.g) * sizeof(unsigned short);
       for (int k = 0; k < 2 && rc == H3D_OK; ++k) {
@@ -212,16 +313,10 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
     he = hipMemcpy(e->d_mem, host.data(), sizeof(EMember) * count, hipMemcpyHostToDevice);
   if (he != hipSuccess) { ensemble_free(e); return (int)he; }
   const int zc = (tune && tune->zchunk > 0) ? tune->zchunk : ENS_ZCHUNK;
-  rc = build_work(e, zc);
+  rc = build_work(e, zc, kb);
   if (rc != H3D_OK) { ensemble_free(e); return rc; }
   *out = e;
   return H3D_OK;
-}
-
-static int ensemble_launch(h3d_ensemble *e, int parity, hipStream_t s) {
-  hipLaunchKernelGGL(ensemble_step, dim3(e->nblocks), dim3(ENS_WAVES * 64),
-                     ENS_LDS, s, e->d_mem, e->d_work, e->nwork, parity, e->zchunk);
-  return h3d_launch_status();
 }
 
 extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
@@ -230,15 +325,12 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
   H3D_TRY(hipSetDevice(e->device));
-  const int parity0 = (int)(e->done % 2);
+  const int p64 = (int)(e->done % 2), p32 = (int)(e->passes % 2);
   if (graph && steps > 0 &&
-      (e->graph_steps != steps || e->graph_parity != parity0 || !e->graph)) {
-    if (e->graph) { (void)hipGraphExecDestroy(e->graph); e->graph = nullptr; }
-    if (e->graph_def) { (void)hipGraphDestroy(e->graph_def); e->graph_def = nullptr; }
+      (e->graph_steps != steps || e->graph_p64 != p64 || e->graph_p32 != p32 || !e->graph)) {
+    drop_graph(e);
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = H3D_OK;
-    for (int k = 0; k < steps && rc == H3D_OK; ++k)
-      rc = ensemble_launch(e, (parity0 + k) % 2, s);
+    int rc = ens_issue(e, steps, p64, p32, s);
     hipGraph_t gdef = nullptr;
     hipError_t ec = hipStreamEndCapture(s, &gdef);
     if (rc != H3D_OK) { if (gdef) (void)hipGraphDestroy(gdef); return rc; }
@@ -246,89 +338,100 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
     e->graph_def = gdef;
     H3D_TRY(hipGraphInstantiate(&e->graph, gdef, nullptr, nullptr, 0));
     e->graph_steps = steps;
-    e->graph_parity = parity0;
+    e->graph_p64 = p64;
+    e->graph_p32 = p32;
   }
-  hipEvent_t e0, e1;
-  H3D_TRY(hipEventCreate(&e0));
-  H3D_TRY(hipEventCreate(&e1));
-  int rc = H3D_OK;
-  hipError_t he = hipEventRecord(e0, s);
-  if (he != hipSuccess) rc = (int)he;
-  if (rc == H3D_OK && steps > 0) {
+  h3d_run_res R;
+  H3D_TRY(hipEventCreate(&R.e0));
+  H3D_TRY(hipEventCreate(&R.e1));
+  H3D_TRY(hipEventRecord(R.e0, s));
+  if (steps > 0) {
     if (graph) {
-      he = hipGraphLaunch(e->graph, s);
-      if (he != hipSuccess) rc = (int)he;
+      H3D_TRY(hipGraphLaunch(e->graph, s));
     } else {
-      for (int k = 0; k < steps && rc == H3D_OK; ++k)
-        rc = ensemble_launch(e, (parity0 + k) % 2, s);
+      const int rc = ens_issue(e, steps, p64, p32, s);
+      if (rc != H3D_OK) return rc;
     }
   }
-  if (rc == H3D_OK) {
-    he = hipEventRecord(e1, s);
-    if (he == hipSuccess) he = hipEventSynchronize(e1);
-    float t = 0.f;
-    if (he == hipSuccess) he = hipEventElapsedTime(&t, e0, e1);
-    if (he != hipSuccess) rc = (int)he;
-    else if (ms) *ms = t;
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  if (rc == H3D_OK) e->done += steps;
-  return rc;
+  H3D_TRY(hipEventRecord(R.e1, s));
+  H3D_TRY(hipEventSynchronize(R.e1));
+  float t = 0.f;
+  H3D_TRY(hipEventElapsedTime(&t, R.e0, R.e1));
+  if (ms) *ms = t;
+  e->done += steps;
+  e->passes += ens_passes(e, steps);
+  return H3D_OK;
 }
 
-// Times each candidate z-march depth over a few launches and keeps the
-// fastest.  Every timed launch has the parity of the ensemble's next step,
-// so it reads the current fields and rewrites the other buffer with the
-// step the next run computes again: member state is unchanged.
+// Times candidate z-march depths and, when the ensemble has FP32 members,
+// FP32 pass depths 1..3, and keeps the fastest per step.  A depth-K group
+// costs one launch over every tile plus K-1 launches over the FP64 tiles;
+// each is timed on its own with the parities of the ensemble's next step,
+// so it reads the current fields and rewrites only the idle buffers with
+// values the next run recomputes: member state is unchanged.
 extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *out) {
   if (!e) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   H3D_TRY(hipSetDevice(e->device));
-  const int parity = (int)(e->done % 2);
-  hipEvent_t e0, e1;
-  H3D_TRY(hipEventCreate(&e0));
-  H3D_TRY(hipEventCreate(&e1));
-  int best_zc = e->zchunk, rc = H3D_OK;
-  float best = 1e30f;
-  for (int zc : {32, 64, 128, 256}) {
-    rc = build_work(e, zc);
-    if (rc != H3D_OK) break;
-    rc = ensemble_launch(e, parity, s);  // warm-up
-    float t = 1e30f;
+  const int p64 = (int)(e->done % 2), p32 = (int)(e->passes % 2);
+  bool has32 = false;
+  for (const auto &t : e->tiles) has32 = has32 || t.precision == 1;
+  h3d_run_res R;
+  H3D_TRY(hipEventCreate(&R.e0));
+  H3D_TRY(hipEventCreate(&R.e1));
+  auto time_list = [&](int list, float *best) {
+    int rc = ens_launch(e, list, p64, p32, s);  // warm-up
+    *best = 1e30f;
     for (int r = 0; r < 3 && rc == H3D_OK; ++r) {
       float ms = 0.f;
-      hipError_t he = hipEventRecord(e0, s);
-      if (he == hipSuccess) rc = ensemble_launch(e, parity, s);
+      hipError_t he = hipEventRecord(R.e0, s);
+      if (he == hipSuccess) rc = ens_launch(e, list, p64, p32, s);
       if (rc != H3D_OK) break;
-      if (he == hipSuccess) he = hipEventRecord(e1, s);
-      if (he == hipSuccess) he = hipEventSynchronize(e1);
-      if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
-      if (he != hipSuccess) { rc = (int)he; break; }
-      t = std::min(t, ms);
+      if (he == hipSuccess) he = hipEventRecord(R.e1, s);
+      if (he == hipSuccess) he = hipEventSynchronize(R.e1);
+      if (he == hipSuccess) he = hipEventElapsedTime(&ms, R.e0, R.e1);
+      if (he != hipSuccess) return (int)he;
+      *best = std::min(*best, ms);
     }
-    if (rc != H3D_OK) break;
-    if (t < best) { best = t; best_zc = zc; }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+    return rc;
+  };
+  int best_zc = e->zchunk, best_kb = e->kb, rc = H3D_OK;
+  float best = 1e30f;
+  for (int kb = 1; kb <= (has32 ? ENS_MAXKB : 1) && rc == H3D_OK; ++kb)
+    for (int zc : {32, 64, 128, 256}) {
+      rc = build_work(e, zc, kb);
+      if (rc != H3D_OK) break;
+      float tg = 0.f, t64 = 0.f;
+      rc = time_list(0, &tg);
+      if (rc == H3D_OK && kb > 1) rc = time_list(2, &t64);
+      if (rc != H3D_OK) break;
+      const float per_step = (tg + (kb - 1) * t64) / kb;
+      if (per_step < best) { best = per_step; best_zc = zc; best_kb = kb; }
+    }
   if (rc != H3D_OK) return rc;
-  rc = build_work(e, best_zc);
+  rc = build_work(e, best_zc, best_kb);
   if (rc != H3D_OK) return rc;
   if (out) {
     out->rows = 0;
     out->waves = ENS_WAVES;
     out->zchunk = best_zc;
-    out->kblock = 1;
+    out->kblock = best_kb;
   }
+  return H3D_OK;
+}
+
+extern "C" int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b) {
+  if (!e || !in_b || member < 0 || member >= e->count) return H3D_EARG;
+  const long long flips = e->tiles[member].precision ? e->passes : e->done;
+  *in_b = (int)(flips % 2);
   return H3D_OK;
 }
 
 extern "C" int h3d_ensemble_info(const h3d_ensemble *e, int *nwork,
                                  int *nblocks, long long *steps_done) {
   if (!e) return H3D_EARG;
-  if (nwork) *nwork = e->nwork;
-  if (nblocks) *nblocks = e->nblocks;
+  if (nwork) *nwork = e->n_kb;
+  if (nblocks) *nblocks = e->blocks_kb;
   if (steps_done) *steps_done = e->done;
   return H3D_OK;
 }

@@ -104,6 +104,7 @@ SIGNATURES = {
     "h3d_ensemble_create": (_I, [C.POINTER(_P), C.POINTER(Member), _I, _TP, _I]),
     "h3d_ensemble_run": (_I, [_P, _I, _I, _P, C.POINTER(C.c_float)]),
     "h3d_ensemble_autotune": (_I, [_P, _P, _TP]),
+    "h3d_ensemble_where": (_I, [_P, _I, C.POINTER(_I)]),
     "h3d_ensemble_destroy": (_I, [_P]),
     "h3d_ensemble_info": (_I, [_P, C.POINTER(_I), C.POINTER(_I), C.POINTER(C.c_longlong)]),
 }

@@ -37,6 +37,19 @@ def build(h3d, specs):
     return e, keep
 
 
+def where(h3d, e, m):
+    in_b = C.c_int()
+    h3d.call("h3d_ensemble_where", e, m, C.byref(in_b))
+    return in_b.value
+
+
+def pass_depths(steps, kb):
+    """FP32 pass depths of one run of `steps` steps at temporal-blocking
+    depth kb: full groups of kb, then single steps."""
+    full = steps // kb if kb > 1 else 0
+    return [kb] * full + [1] * (steps - full * kb)
+
+
 def run(h3d, e, steps, graph, stream):
     ms = C.c_float()
     h3d.call("h3d_ensemble_run", e, steps, graph, stream.ptr, C.byref(ms))
@@ -96,8 +109,8 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
     assert out.zchunk in (32, 64, 128, 256)
     run(gpu, e, 2, 1, s)
     steps = 5
-    for sp, (da, db, dk) in zip(specs, keep):
-        got = db.download()
+    for m, (sp, (da, db, dk)) in enumerate(zip(specs, keep)):
+        got = (db if where(gpu, e, m) else da).download()
         g = sp["g"]
         if sp["precision"] == 0:
             want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
@@ -108,6 +121,48 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
             want, _ = oracle.run_f32(sp["u"], sp["k"], None, g.nx, g.ny, g.nz, f32, steps,
                                      1, 2)
             assert got.tobytes() == want.tobytes(), g.nx
+    gpu.call("h3d_ensemble_destroy", e)
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("kb", [2, 3])
+def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
+    """FP32 members fused kb steps per pass (FP64 members stepping alone in
+    between): each FP32 member equals the oracle replaying the same passes
+    with the FP16 residual carried across them, and each FP64 member its
+    plain run, over runs whose lengths are not multiples of kb."""
+    specs = [make_member(gpu, d, m, 13, p) for m, (d, p) in enumerate(SPECS)]
+    mems, keep = [], []
+    for sp in specs:
+        da, db, dk = (gpu.DeviceArray.from_host(x) for x in (sp["u"], sp["u"], sp["k"]))
+        keep.append((da, db, dk))
+        mems.append(gpu.Member(da.ptr, db.ptr, dk.ptr, sp["g"], sp["factor"], sp["precision"]))
+    arr = (gpu.Member * len(mems))(*mems)
+    e = C.c_void_p()
+    t = gpu.Tune(0, 0, 0, kb)
+    gpu.call("h3d_ensemble_create", C.byref(e), arr, len(mems), C.byref(t), 0)
+    s = gpu.Stream()
+    runs = [3, 4, kb]
+    for n in runs:
+        run(gpu, e, n, graph, s)
+    depths = [d for n in runs for d in pass_depths(n, kb)]
+    steps = sum(runs)
+    for m, (sp, (da, db, dk)) in enumerate(zip(specs, keep)):
+        got = (db if where(gpu, e, m) else da).download()
+        g = sp["g"]
+        if sp["precision"] == 0:
+            assert where(gpu, e, m) == steps % 2
+            want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
+                                  g.nz, steps, factor=sp["factor"])
+            assert np.array_equal(got, want), g.nx
+        else:
+            assert where(gpu, e, m) == len(depths) % 2
+            f32 = float(np.float32(sp["factor"]))
+            u, comp = sp["u"], None
+            for d in depths:
+                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
+                                         comp0=comp)
+            assert got.tobytes() == u.tobytes(), (g.nx, kb)
     gpu.call("h3d_ensemble_destroy", e)
 
 
