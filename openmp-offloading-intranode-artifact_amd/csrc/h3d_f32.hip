@@ -2,11 +2,12 @@
 // on gfx950: one HBM pass advances the field KB time steps.
 //
 // Arithmetic of one cell update (compiled with -ffp-contract=off, so every
-// operation rounds as written and the CPU oracle can replay it bit for bit):
+// operation rounds as written and the CPU oracle can replay it bit for bit;
+// the one fused multiply-add is explicit):
 //   S = ((((Fx+ - Fx-) + Fy+) - Fy-) + Fz+) - Fz-,  F = (k_a + k_b)(u_a - u_b)
-//   d = factor * S
-//   plain:       u' = u + d
-//   compensated: y = d + e; t = u + y; e' = y - (t - u); u' = t   (Kahan)
+//   plain:       u' = fma(factor, S, u)
+//   compensated: y = fma(factor, S, e); t = u + y; e' = y - (t - u); u' = t
+//                (Kahan)
 // The six products are the reference's terms A..F (heat3D_variable_coeff.c:
 // 174-181) in FP32.  A face flux computed for one cell is reused, unchanged,
 // as the opposite face of its neighbour, which is the same expression with

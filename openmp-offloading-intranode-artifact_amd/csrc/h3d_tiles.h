@@ -348,14 +348,15 @@ struct Pass {
         const float fzm = FZ[l][r];
         FZ[l][r] = fzp;
         const float s = ((((fxp - fxm) + fyp) - fym) + fzp) - fzm;
-        const float d = factor * s;
         const bool inside = lane_int && row_int[r] && q_int;
         float res, e2 = 0.0f;
         if (MODE == 0) {
-          res = u + d;
+          res = __builtin_fmaf(factor, s, u);
         } else {
-          // Kahan: carry the part of the increment the FP32 sum dropped
-          const float y = d + E[l - 1][P][r];
+          // Kahan: carry the part of the increment the FP32 sum dropped;
+          // the increment factor*s joins the carried residual in one fused
+          // multiply-add (one rounding)
+          const float y = __builtin_fmaf(factor, s, E[l - 1][P][r]);
           const float t = u + y;
           e2 = y - (t - u);
           res = t;
@@ -595,12 +596,11 @@ __device__ __forceinline__ void march_f32_tile(
       const float s = (kip + kk) * (uip - u) - (kk + kim) * (u - uim) +
                       (kjp + kk) * (ujp - u) - (kk + kjm) * (u - ujm) +
                       (kq[r] + kk) * (up[r] - u) - (kk + km[r]) * (u - um[r]);
-      const float d = factor * s;
       if (MODE == 0) {
-        st_f32(ro4, b4o(so[r]), u + d);
+        st_f32(ro4, b4o(so[r]), __builtin_fmaf(factor, s, u));
       } else {
         const float e = (MODE == 2) ? comp_decode(cc[r], u) : 0.0f;
-        const float y = d + e;
+        const float y = __builtin_fmaf(factor, s, e);
         const float t = u + y;
         st_f32(ro4, b4o(so[r]), t);
         if (MODE == 2) st_u16(rc2, b2o(so[r]), comp_encode(y - (t - u), t));

@@ -22,10 +22,11 @@
  *       decomposition (n=1026 -> 258 planes) and slab-vs-global equality;
  *   orc_metrics -- hand-computed vectors;
  *   orc_stats, orc_face_sum_max -- math.fsum / brute force in Python;
- *   orc_run_f32 -- FP32 paths have no reference counterpart.  Mode 0 is
- *       pinned against a numpy float32 restatement; modes 1-2 against the
- *       FP64 oracle (error bounds) and against their own invariants
- *       (kb=1 mode 1 == plain when the residual never matters, ...).
+ *   orc_run_f32 -- FP32 paths have no reference counterpart.  All modes
+ *       are pinned against a numpy float32 restatement (whose fused
+ *       multiply-add is an exact FP64 emulation, itself pinned against
+ *       rational arithmetic), and modes 1-2 against the FP64 oracle
+ *       (error bounds).
  */
 #include "h3d_oracle.h"
 
@@ -432,12 +433,13 @@ void orc_run_f32(float *ua, float *ub, const float *kappa_f32,
                       (kc + kap[c - sy]) * (uc - uo[c - sy]) +
                       (kap[c + sz] + kc) * (uo[c + sz] - uc) -
                       (kc + kap[c - sz]) * (uc - uo[c - sz]);
-            float d = factor * S;
             if (mode == 0) {
-              un[c] = uc + d;
+              un[c] = fmaf(factor, S, uc);
             } else {
-              /* Kahan: keep the part of y the FP32 sum dropped */
-              float y = d + err[c];
+              /* Kahan: keep the part of y the FP32 sum dropped; the
+               * increment factor*S joins the residual in one fused
+               * multiply-add (a single rounding) */
+              float y = fmaf(factor, S, err[c]);
               float t = uc + y;
               err[c] = y - (t - uc);
               un[c] = t;

@@ -96,9 +96,9 @@ int orc_write_dump(const char *fname, const double *u, int n);
  *
  * One step of cell c with stored value u, carried residual e:
  *   S  = ((((A - B) + C) - D) + E) - F   (A..F as in orc_step_f64, FP32)
- *   d  = f32factor * S
- *   y  = d + e ;  t = u + y ;  e' = y - (t - u) ;  u' = t   (Kahan)
- * plain FP32 (mode 0) uses u' = u + d and no e.
+ *   y  = fma(f32factor, S, e) ;  t = u + y ;  e' = y - (t - u) ;  u' = t
+ *   (Kahan, with the increment joining the residual in one rounding)
+ * plain FP32 (mode 0) uses u' = fma(f32factor, S, u) and no e.
  * kappa is given as float (fp32 mode) or as bf16 bits times scale
  * (bf16 mode: kappa = bf16_to_f32(bits) and f32factor already carries
  * the scale).  Residual modes:

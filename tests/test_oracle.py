@@ -269,13 +269,97 @@ def f32_problem(nx, ny, nz, seed=3):
     return u.ravel(), k.ravel()
 
 
+def np_fmaf(a, b, c):
+    """float32 fused multiply-add a*b + c with one rounding, emulated in
+    float64: the product of two float32 values is exact in float64; the
+    sum is rounded to float64 and its error recovered exactly (TwoSum); a
+    float64 sum that sits on a float32 rounding midpoint is resolved by the
+    sign of that error (the only case where rounding twice differs from
+    rounding once)."""
+    a = np.asarray(a, np.float32).astype(np.float64)
+    b = np.asarray(b, np.float32).astype(np.float64)
+    c = np.asarray(c, np.float32).astype(np.float64)
+    p = a * b
+    s = p + c
+    bb = s - p
+    err = (p - (s - bb)) + (c - bb)
+    r = s.astype(np.float32)
+    r64 = r.astype(np.float64)
+    away = np.where(r64 < s, np.float32(np.inf), np.float32(-np.inf)).astype(np.float32)
+    o = np.nextafter(r, away)
+    o64 = o.astype(np.float64)
+    tie = (r64 != s) & ((s - r64) == (o64 - s))
+    take_o = tie & (err != 0) & ((err > 0) == (o64 > s))
+    return np.where(take_o, o, r).astype(np.float32)
+
+
+def f32_S(u, k):
+    """S of the FP32 update in the reference's term order, rounded per
+    operation in float32."""
+    c = u[1:-1, 1:-1, 1:-1]
+    kc = k[1:-1, 1:-1, 1:-1]
+
+    def nb(a, dz, dy, dx):
+        n0, n1, n2 = a.shape
+        return a[1 + dz:n0 - 1 + dz, 1 + dy:n1 - 1 + dy, 1 + dx:n2 - 1 + dx]
+
+    return ((nb(k, 0, 0, 1) + kc) * (nb(u, 0, 0, 1) - c)
+            - (kc + nb(k, 0, 0, -1)) * (c - nb(u, 0, 0, -1))
+            + (nb(k, 0, 1, 0) + kc) * (nb(u, 0, 1, 0) - c)
+            - (kc + nb(k, 0, -1, 0)) * (c - nb(u, 0, -1, 0))
+            + (nb(k, 1, 0, 0) + kc) * (nb(u, 1, 0, 0) - c)
+            - (kc + nb(k, -1, 0, 0)) * (c - nb(u, -1, 0, 0)))
+
+
 def numpy_f32_plain(u, k, factor, steps, shape):
-    u = u.reshape(shape).copy()
-    k = k.reshape(shape)
+    u = u.reshape(shape).astype(np.float32).copy()
+    k = k.reshape(shape).astype(np.float32)
     f = np.float32(factor)
     for _ in range(steps):
-        u = numpy_step(u, k, np.zeros_like(u), f).astype(np.float32)
+        S = f32_S(u, k)
+        u = u.copy()
+        u[1:-1, 1:-1, 1:-1] = np_fmaf(f, S, u[1:-1, 1:-1, 1:-1])
     return u.ravel()
+
+
+def _fma_exact(a, b, c):
+    """a*b + c rounded once to float32, from exact rational arithmetic."""
+    from fractions import Fraction
+    v = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    r = np.float32(float(v))  # candidate; fix by comparing exact distances
+    best = r
+    for cand in (np.nextafter(r, np.float32(-np.inf)), r, np.nextafter(r, np.float32(np.inf))):
+        d, db = abs(Fraction(float(cand)) - v), abs(Fraction(float(best)) - v)
+        if d < db or (d == db and (cand.view(np.uint32) & 1) == 0 and cand != best):
+            best = cand
+    return best
+
+
+def test_np_fmaf_rounds_once():
+    """The float64 emulation of the FP32 fused multiply-add against exact
+    rational arithmetic and against the C library's fmaf, on random
+    operands and on a constructed case where rounding twice (to float64,
+    then to float32) lands on a tie and goes the wrong way."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.fmaf.restype = ctypes.c_float
+    libm.fmaf.argtypes = [ctypes.c_float] * 3
+    rng = np.random.default_rng(11)
+    a = (rng.standard_normal(400) * 10.0 ** rng.integers(-3, 3, 400)).astype(np.float32)
+    b = (rng.standard_normal(400) * 10.0 ** rng.integers(-3, 3, 400)).astype(np.float32)
+    c = (-a.astype(np.float64) * b * (1 + rng.standard_normal(400) * 1e-6)).astype(np.float32)
+    # the double-rounding trap: a*b + c = 1 + 2^-23 + 2^-24 - 2^-70 exactly
+    trap = (np.float32(2.0 ** -12 * (1 + 2.0 ** -23)), np.float32(2.0 ** -12 * (1 - 2.0 ** -23)),
+            np.float32(1 + 2.0 ** -23))
+    a, b, c = (np.append(x, t) for x, t in zip((a, b, c), trap))
+    got = np_fmaf(a, b, c)
+    for i in range(a.size):
+        want = _fma_exact(a[i], b[i], c[i])
+        assert got[i].view(np.uint32) == want.view(np.uint32), (a[i], b[i], c[i])
+        assert np.float32(libm.fmaf(a[i], b[i], c[i])).view(np.uint32) == want.view(np.uint32)
+    assert got[-1] == np.float32(1 + 2.0 ** -23)  # not the tie's even neighbour
+    naive = (a[-1].astype(np.float64) * b[-1] + c[-1]).astype(np.float32)
+    assert naive != got[-1]
 
 
 def test_f32_plain_matches_numpy(oracle):
@@ -325,8 +409,7 @@ def numpy_f32_compensated(u, k, factor, steps, kb, mode, shape):
                  - (kc + nb(k, 0, -1, 0)) * (c - nb(u, 0, -1, 0))
                  + (nb(k, 1, 0, 0) + kc) * (nb(u, 1, 0, 0) - c)
                  - (kc + nb(k, -1, 0, 0)) * (c - nb(u, -1, 0, 0)))
-            d = f * S
-            y = d + e[inner]
+            y = np_fmaf(f, S, e[inner])
             t = c + y
             e[inner] = y - (t - c)
             u = u.copy()
@@ -351,12 +434,14 @@ def test_f32_compensated_matches_numpy(oracle, mode, kb):
 
 
 def test_f32_compensated_is_closer_to_f64(oracle):
+    # small increments (factor*S << u), the regime compensation is for: the
+    # rounding of u + d then dominates the plain update's error
     nx, ny, nz = 12, 11, 10
     u, k = f32_problem(nx, ny, nz, seed=9)
     u64 = u.astype(np.float64)
     k64 = k.astype(np.float64)
-    factor = np.float32(0.04)
-    steps = 400
+    factor = np.float32(0.004)
+    steps = 2000
     ref = oracle.run_f64(u64.copy(), u64.copy(), None, k64, nx, ny, nz, steps,
                          factor=float(factor))
     err = {}
