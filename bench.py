@@ -382,9 +382,9 @@ def leg_cfg5(bw, stream):
 def main():
     a = parse()
     if a.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (--gpus {a.gpus} requested); "
-              "the z-slab path would shard across GPUs as described in DESIGN.md, "
-              "but no multi-GPU figure is produced.", file=sys.stderr)
+        print(f"bench.py: one GPU only (--gpus {a.gpus} requested): the update would "
+              "shard as z-slabs, one per GPU, exchanging one face per neighbour per "
+              "step, but no multi-GPU figure is produced.", file=sys.stderr)
         sys.exit(2)
     if h3d.device_count() < 1:
         print("bench.py: no HIP device", file=sys.stderr)
