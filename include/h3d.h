@@ -306,6 +306,12 @@ int h3d_init_partition(double *u, double *rhs, double *kappa, int nz_local,
  * (parts = 2): nz_d[p] planes incl. 2 ghosts, first owned global plane
  * gk_start[p].  H3D_EARG if a slab has fewer than 4 planes. */
 int h3d_slab_decompose(int n, int parts, int *nz_d, int *gk_start);
+/* 4-openmp-stream-omp-p2p.c:776-820: copies the planes slab `part` of a
+ * `parts`-way split owns (global k = gk_start[part] .. gk_start[part+1]-1,
+ * plus k = 0 for the first and k = n-1 for the last slab) from its dense
+ * slab array (nz_d[part] planes of n*n) into the global n^3 array. */
+int h3d_slab_reconstruct(double *global, const double *slab, int n, int parts,
+                         int part);
 
 /* Seeded inputs (no reference counterpart: the reference has no RNG).
  * Interior m^3 of a global grid of (m+2)^3 points, h = 1/(m+1), x = i*h.

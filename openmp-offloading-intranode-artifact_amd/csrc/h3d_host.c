@@ -124,6 +124,29 @@ int h3d_slab_decompose(int n, int parts, int *nz_d, int *gk_start)
   return H3D_OK;
 }
 
+/* 4-openmp-stream-omp-p2p.c:776-820: slab `part` contributes its owned
+ * planes to the global [k][j][i] array; interface ghosts are left to their
+ * owners, and the first and last slab also contribute the physical
+ * boundary planes k = 0 and k = n-1. */
+int h3d_slab_reconstruct(double *global, const double *slab, int n, int parts,
+                         int part)
+{
+  int nz_d[4], gk_start[4];
+  if (!global || !slab || parts < 1 || parts > 4 || part < 0 || part >= parts)
+    return H3D_EARG;
+  int rc = h3d_slab_decompose(n, parts, nz_d, gk_start);
+  if (rc != H3D_OK) return rc;
+  const int lo = part == 0 ? 0 : gk_start[part];
+  const int hi = part == parts - 1 ? n - 1 : gk_start[part + 1] - 1;
+  const size_t plane = (size_t)n * (size_t)n;
+  for (int k = 0; k < nz_d[part]; k++) {
+    const int gk = k - 1 + gk_start[part];
+    if (gk >= lo && gk <= hi)
+      memcpy(global + (size_t)gk * plane, slab + (size_t)k * plane, plane * sizeof(double));
+  }
+  return H3D_OK;
+}
+
 /* ---- seeded inputs ------------------------------------------------------ */
 
 static inline uint64_t mix64(uint64_t z)

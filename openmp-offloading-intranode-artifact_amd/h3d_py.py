@@ -96,6 +96,7 @@ SIGNATURES = {
     "h3d_init_reference": (_I, [_P, _P, _P, _GP]),
     "h3d_init_partition": (_I, [_P, _P, _P, _I, _I, _I, _I, _I, _I, _D, _D]),
     "h3d_slab_decompose": (_I, [_I, _I, C.POINTER(_I), C.POINTER(_I)]),
+    "h3d_slab_reconstruct": (_I, [_P, _P, _I, _I, _I]),
     "h3d_gen_seeded": (_I, [_P, _P, _GP, _I, _I, _I, _I, _I, C.c_uint64, _I]),
     "h3d_rng_uniform": (_D, [C.c_uint64, _I, _I, C.c_uint64]),
     "h3d_face_sum_max_host": (_D, [_P, _GP]),

@@ -2,6 +2,7 @@
 symbol of include/h3d.h is exported, and the host-side set-up functions
 agree with the oracle (reference mode) or with an independent numpy
 restatement (seeded generators, which the reference does not have)."""
+import ctypes as C
 import os
 import re
 import subprocess
@@ -204,3 +205,41 @@ def test_verify_tool_exit_codes_without_device(tmp_path):
     (tmp_path / "r.txt").write_text("1\n2\n")
     r = subprocess.run([exe, "--ref", "r.txt"], cwd=tmp_path, capture_output=True, text=True)
     assert r.returncode == 2 and "not n^3" in r.stdout
+
+
+@pytest.mark.parametrize("n,parts", [(9, 2), (10, 2), (11, 4), (22, 4)])
+def test_slab_reconstruct_is_the_reference_assembly(h3d, n, parts):
+    """Each slab holds global planes gk_start-1 .. gk_start+nz_d-2; the
+    reconstruction takes the owned ones plus the physical boundary planes
+    (4-openmp-stream-omp-p2p.c:776-820) and so rebuilds the global array
+    exactly, whichever slab writes first; interface ghosts come from the
+    slab that owns them."""
+    L = h3d.lib()
+    nz_d, gks = (C.c_int * 4)(), (C.c_int * 4)()
+    assert L.h3d_slab_decompose(n, parts, nz_d, gks) == 0
+    nn = n * n
+    glob = np.arange(n * nn, dtype=np.float64)
+    out = np.full(n * nn, -1.0)
+    for p in reversed(range(parts)):
+        planes = [gks[p] - 1 + k for k in range(nz_d[p])]
+        slab = np.concatenate([glob[g * nn:(g + 1) * nn] for g in planes])
+        # ghost planes hold stale data in a real run: make them wrong here
+        if p > 0:
+            slab[:nn] = -7.0
+        if p < parts - 1:
+            slab[-nn:] = -7.0
+        assert L.h3d_slab_reconstruct(h3d.ptr(out), h3d.ptr(slab), n, parts, p) == 0
+    assert np.array_equal(out, glob)
+    assert L.h3d_slab_reconstruct(h3d.ptr(out), h3d.ptr(glob), n, parts, parts) == -1
+    assert L.h3d_slab_reconstruct(h3d.ptr(out), h3d.ptr(glob), n, 3, 0) == -1
+
+
+def test_slab_reconstruct_plane_ranges_n1026(h3d):
+    """At the slab configuration's size (n = 1026, 4 slabs of 258 planes)
+    each slab contributes 256 owned planes, the outer ones one boundary
+    plane more: ranges 0..256, 257..512, 513..768, 769..1025."""
+    L = h3d.lib()
+    nz_d, gks = (C.c_int * 4)(), (C.c_int * 4)()
+    assert L.h3d_slab_decompose(1026, 4, nz_d, gks) == 0
+    assert list(nz_d) == [258] * 4 and list(gks) == [1, 257, 513, 769]
+
