@@ -43,6 +43,24 @@ def test_full_field_bitwise_vs_oracle(gpu, oracle, big):
     assert m[4] <= 1e-12
 
 
+def test_1000_steps_vs_oracle_subsample_and_checksum(gpu, oracle, big):
+    """The configuration's own check: 1000 FP64 steps at 512^3 against the
+    CPU oracle (OpenMP on the host) -- max relative error <= 1e-12 on the
+    64^3 strided subsample and the full-field checksum; both are exact
+    here, as the kernel reproduces the reference's arithmetic."""
+    steps = 1000
+    res, _ = run64(gpu, big, big["u"], big["k"], steps)
+    got = res.download()
+    want = oracle.run_f64(big["u"].copy(), big["u"].copy(), None, big["k"], N, N, N,
+                          steps, factor=big["factor"], nthreads=16)
+    sub = (slice(1, N - 1, 8),) * 3
+    m = oracle.metrics(want.reshape(N, N, N)[sub].ravel(), got.reshape(N, N, N)[sub].ravel())
+    assert m[4] <= 1e-12 and m[0] == 0.0
+    assert np.array_equal(got, want)  # the whole field, hence its checksum
+    assert gpu.stats(res, big["g"])[0] == pytest.approx(oracle.stats(want, N, N, N)[0],
+                                                        rel=1e-14)
+
+
 def test_1000_steps_invariants(gpu, big):
     g = big["g"]
     u0 = big["u"]
