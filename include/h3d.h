@@ -193,10 +193,11 @@ int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
                 int *final_in_b);
 
 /* Choose tune fields for a problem by timing candidate shapes on the
- * device (a few passes each, on scratch copies of the caller's buffers);
- * results are cached per (kernel, nx, ny, nz, mode, kappa format).
- * kind: 0 = FP64 step, 1 = FP32 pass (kblock chosen too unless
- * fixed_kblock > 0).  The buffers' contents are not modified. */
+ * device (a few passes each, reading u_a and kappa and writing a scratch
+ * buffer the tuner allocates; u_b is not used); results are cached per
+ * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step,
+ * 1 = FP32 pass (kblock chosen too, 1..4, unless fixed_kblock > 0).  The
+ * caller's buffers are not modified. */
 int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
                  int kappa_bf16, const h3d_grid *g, int mode,
                  int fixed_kblock, void *stream, h3d_tune *out);
