@@ -141,3 +141,30 @@ def test_pass_argument_errors(gpu):
     t = gpu.Tune(2, 2, 0, 4)  # 4 rows per workgroup cannot hold a 2*4 halo
     assert L.h3d_pass_f32(d1.ptr, d2.ptr, d2.ptr, None, None, None, gpu.C.byref(g), f, 4, 0,
                           gpu.C.byref(t), None) == -1
+
+
+@pytest.mark.parametrize("kb", [1, 2])
+def test_largest_plane_stride_bitwise(gpu, oracle, kb):
+    """Planes of 2^28 elements (1 GiB of FP32), the largest the 32-bit
+    in-plane byte offsets allow, for the single-step and the fused tile:
+    bit-identical to the same values stored densely."""
+    from test_gpu_f64 import get_rows, put_rows
+    nx, ny, nz = 70, 9, 5
+    sy, sz = 72, 1 << 28
+    P = problem(gpu, nx, ny, nz)
+    steps = 2 * kb
+    want, _ = oracle.run_f32(P["u32"], P["k32"], None, nx, ny, nz,
+                             float(np.float32(P["factor"])), steps, kb, 1)
+    g = gpu.Grid(nx, ny, nz, sy, sz)
+    bufs = [gpu.DeviceArray(sz * nz * 4, dtype=np.float32) for _ in range(3)]
+    for b, a in zip(bufs, (P["u32"], P["u32"], P["k32"])):
+        put_rows(gpu, b, a, nx, ny, nz, sy, sz)
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    t = gpu.Tune(0, 0, 0, kb)
+    gpu.call("h3d_run_f32", bufs[0].ptr, bufs[1].ptr, bufs[2].ptr, None, None, None,
+             gpu.C.byref(g), gpu.C.c_float(np.float32(P["factor"])), steps, 1, gpu.C.byref(t),
+             0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = get_rows(gpu, bufs[1] if fb.value else bufs[0], np.float32, nx, ny, nz, sy, sz)
+    assert got.tobytes() == want.tobytes()
+    for b in bufs:
+        b.close()

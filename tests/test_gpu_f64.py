@@ -164,3 +164,49 @@ def test_argument_errors(gpu):
     assert L.h3d_heat_update(e.ptr, d.ptr, None, d.ptr, 10, 10, 10, 0.1, 7) == -3
     # an empty range is a no-op, as the reference's loops are
     assert L.h3d_step_f64(e.ptr, d.ptr, None, d.ptr, gpu.C.byref(g), 0.1, 5, 4, None, None) == 0
+
+
+def put_rows(gpu, buf, a, nx, ny, nz, sy, sz):
+    """Copy dense [k][j][i] rows of `a` into a strided device buffer."""
+    item = a.itemsize
+    for kk in range(nz):
+        for j in range(ny):
+            row = np.ascontiguousarray(a[(kk * ny + j) * nx:(kk * ny + j + 1) * nx])
+            dst = gpu.C.c_void_p(buf.ptr.value + (kk * sz + j * sy) * item)
+            gpu.call("h3d_target_memcpy", dst, gpu.ptr(row), gpu.C.c_size_t(row.nbytes), 0, -1)
+
+
+def get_rows(gpu, buf, dtype, nx, ny, nz, sy, sz):
+    out = np.empty(nx * ny * nz, dtype)
+    item = out.itemsize
+    for kk in range(nz):
+        for j in range(ny):
+            row = np.empty(nx, dtype)
+            src = gpu.C.c_void_p(buf.ptr.value + (kk * sz + j * sy) * item)
+            gpu.call("h3d_target_memcpy", gpu.ptr(row), src, gpu.C.c_size_t(row.nbytes), -1, 0)
+            out[(kk * ny + j) * nx:(kk * ny + j + 1) * nx] = row
+    return out
+
+
+def test_largest_plane_stride_bitwise(gpu, oracle):
+    """Planes of 2^28 elements (2 GiB of FP64), the largest the kernels'
+    32-bit in-plane byte offsets allow: a thin problem laid out with that
+    plane stride steps bit-identically to the same values stored densely
+    (the last planes start 6 GiB into the buffers)."""
+    nx, ny, nz = 70, 5, 4
+    sy, sz = 72, 1 << 28
+    gd = gpu.Grid.dense(nx, ny, nz)
+    u, k = gpu.gen_seeded(gd, 0, nx, ny, nz, 1, 5)
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    want = oracle.run_f64(u.copy(), u.copy(), None, k, nx, ny, nz, 3, factor=factor)
+    g = gpu.Grid(nx, ny, nz, sy, sz)
+    bufs = [gpu.DeviceArray(sz * nz * 8) for _ in range(3)]
+    for b, a in zip(bufs, (u, u, k)):
+        put_rows(gpu, b, a, nx, ny, nz, sy, sz)
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    gpu.call("h3d_run_f64", bufs[0].ptr, bufs[1].ptr, None, bufs[2].ptr, gpu.C.byref(g),
+             factor, 3, None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = get_rows(gpu, bufs[1] if fb.value else bufs[0], np.float64, nx, ny, nz, sy, sz)
+    assert np.array_equal(got, want)
+    for b in bufs:
+        b.close()
