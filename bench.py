@@ -360,8 +360,9 @@ def leg_cfg5(bw, stream):
         got = (db if in_b.value else da).download().astype(np.float64)
         ref = ta.download()  # FP64 twins: 1000 steps, results in u_a
         per_member.append({"member": 2 * mi + 1, "dims": list(dims[2 * mi + 1]),
-                           "rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
-                           "max_abs": float(np.max(np.abs(got - ref)))})
+                           "rel_l2": float("%.3g" % (np.linalg.norm(got - ref)
+                                                     / np.linalg.norm(ref))),
+                           "max_abs": float("%.3g" % np.max(np.abs(got - ref)))})
     worst = max(m["rel_l2"] for m in per_member)
     h3d.call("h3d_ensemble_destroy", e)
     h3d.call("h3d_ensemble_destroy", e2)
