@@ -1,0 +1,52 @@
+#!/usr/bin/env python3
+"""Writes the benchmark-pipeline fixtures from the reference's recorded MI250X
+HIP results (run here, where the reference tree is present; the fixtures are
+committed and the tests never read the reference):
+
+  hip_mi250x_runlog.txt   the lines of every results_*/run.log that
+                          aggregate.sh parses ('>>> Running:' headers,
+                          'Estimated timesteps', 'N timesteps complete',
+                          'Time = S seconds'), files in sorted order, each
+                          introduced by a '# <dir>/run.log' line
+  hip_mi250x_summary.csv  the summary.csv aggregate.sh wrote from them
+  hip_mi250x_timings_header.txt  the header line of run.sh's timings.csv
+
+Source: 3D_Heat_Benchmark/AMD-MI250X/Source-Results/HIP/{results_*/run.log,
+summary.csv, results_*/timings.csv}.
+"""
+import glob
+import os
+import re
+import sys
+
+SRC = "/root/reference/3D_Heat_Benchmark/AMD-MI250X/Source-Results/HIP"
+HERE = os.path.dirname(os.path.abspath(__file__))
+KEEP = re.compile(r"^>>> Running:|Estimated\s+timesteps:|[0-9]+\s+timesteps?\s+complete"
+                  r"|^\s*Time\s*=")
+
+
+def main():
+    logs = sorted(glob.glob(os.path.join(SRC, "results_*", "run.log")))
+    if not logs:
+        sys.exit(f"no run logs under {SRC}")
+    out = []
+    for path in logs:
+        out.append("# " + os.path.relpath(path, SRC))
+        with open(path, errors="replace") as f:
+            out.extend(line.rstrip("\n") for line in f if KEEP.search(line))
+    with open(os.path.join(HERE, "hip_mi250x_runlog.txt"), "w") as f:
+        f.write("\n".join(out) + "\n")
+    with open(os.path.join(SRC, "summary.csv")) as f:
+        summary = f.read()
+    with open(os.path.join(HERE, "hip_mi250x_summary.csv"), "w") as f:
+        f.write(summary)
+    timings = sorted(glob.glob(os.path.join(SRC, "results_*", "timings.csv")))
+    with open(timings[0]) as f:
+        header = f.readline()
+    with open(os.path.join(HERE, "hip_mi250x_timings_header.txt"), "w") as f:
+        f.write(header)
+    print(f"{len(logs)} run logs, {len(out)} lines; summary {summary.count(chr(10))} lines")
+
+
+if __name__ == "__main__":
+    main()

@@ -152,3 +152,22 @@ def test_cli_autotunes_large_grids_with_identical_results(tmp_path):
     assert "Autotuned:" in tuned and "Autotuned:" not in plain
     line = lambda out: next(ln for ln in out.splitlines() if ln.startswith("checksum ="))
     assert line(tuned) == line(plain)
+
+
+def test_pipeline_rows_from_h3d(tmp_path):
+    """The program timed the way the artifact's run.sh times its binaries:
+    timings.csv rows with status ok and the solver's reported seconds, and
+    a summary whose step counts are the step rule's for each (n, T)."""
+    import h3d_pipeline as P
+    import h3d_py as h3d
+    out = tmp_path / "res"
+    P.main(["run", "--sizes", "34", "66", "--reps", "2", "--out", str(out)])
+    rows = [r.split(",") for r in (out / "timings.csv").read_text().splitlines()[1:]]
+    assert len(rows) == 4
+    assert all(r[11] == "ok" and float(r[10]) > 0 for r in rows), rows
+    summary = (out / "summary.csv").read_text().splitlines()
+    assert len(summary) == 3
+    for line, n in zip(summary[1:], (34, 66)):
+        f = line.split(",")
+        want = h3d.count_steps(float(P.sim_time(n)), h3d.params_reference(n)[1])
+        assert f[:4] == ["1-hip-h3d", "1", str(n), "2"] and f[9] == str(want), line
