@@ -149,17 +149,22 @@ def seeded_problem(gx, gy, gz, kind, seed, g=None, gk0=0, member=0):
 def leg_reference_workload(bw, stream):
     """The reference's own GPU benchmark case, for comparison with its
     published numbers: reference-mode fields (sin/cos closed forms, rhs),
-    dense layout, n = 512 points per axis (510^3 interior) and T = 500 dt
-    printed %.7g, which the step rule turns into 501 steps
-    (AMD-MI250X/Source-Results/HIP/summary.csv:2; submit.sh:61-62).  32 B
-    per update (u, rhs, kappa read; u written)."""
+    n = 512 points per axis (510^3 interior) and T = 500 dt printed %.7g,
+    which the step rule turns into 501 steps
+    (AMD-MI250X/Source-Results/HIP/summary.csv:2; submit.sh:61-62).  Device
+    fields in the h3d program's layout (rows padded to 32 elements,
+    interior rows 128-B aligned).  32 B per update (u, rhs, kappa read; u
+    written)."""
     n = 512
     h, dt, factor = h3d.params_reference(n)
     T = float("%.7g" % (500 * dt))
     steps = h3d.count_steps(T, dt)
-    g = h3d.Grid.dense(n)
-    u, r, k = h3d.init_reference(n)
-    da, db, dr, dk = (h3d.DeviceArray.from_host(x) for x in (u, u, r, k))
+    g = h3d.pitched_grid(n, n, n)
+    u, r, k = (np.zeros(g.elems) for _ in range(3))
+    h3d.call("h3d_init_reference", h3d.ptr(u), h3d.ptr(r), h3d.ptr(k), h3d.C.byref(g))
+    da, db, dr, dk = (h3d.AlignedField(g, np.float64) for _ in range(4))
+    for d, x in ((da, u), (db, u), (dr, r), (dk, k)):
+        d.upload(x)
     t = h3d.Tune()
     h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 0, stream.ptr,
              h3d.C.byref(t))
