@@ -59,7 +59,7 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
       march_f64_tile<ENS_F64_ROWS, false>(out, in, nullptr,
                                           static_cast<const double *>(m.kap),
                                           m.nx, m.ny, m.sy, m.sz, m.factor, 1,
-                                          m.nz - 2, zchunk, d.y, d.z, d.w);
+                                          m.nz - 2, zchunk, 1, d.y, d.z, d.w);
     } else {
       const float *in = static_cast<const float *>(parity32 ? m.ub : m.ua);
       float *out = static_cast<float *>(parity32 ? m.ua : m.ub);

@@ -28,19 +28,22 @@ __device__ __forceinline__ void st_f64(__amdgpu_buffer_rsrc_t r, unsigned off, d
 }
 
 // One tile (tx, ty, tz) of the FP64 step: waves x 64 lanes, RY rows per
-// lane, planes k_start + tz*zchunk ... (see h3d_f64.hip).  Wavefronts are
-// independent (no barrier), so a wave with nothing to do returns at once.
-// z-neighbours stay in registers, x-neighbours come from the adjacent lanes
-// (DPP) with lanes 0 and 63 loading the x halo, and each wave loads its y
-// halo rows.  Accesses are buffer instructions; lanes and rows outside the
-// array (loads) or off the interior (stores) carry an out-of-range offset
-// instead of a branch.
+// lane, planes k_start + tz*zchunk ... (see h3d_f64.hip).  Lane l of tile
+// tx holds x = x0 + 64*tx + l; x0 <= 1 is chosen by the caller so that
+// tiles start on 128-byte lines of u_new where the layout allows (x0 = 1:
+// the classic origin; x0 = 0 or negative: lanes left of the interior idle).
+// Wavefronts are independent (no barrier), so a wave with nothing to do
+// returns at once.  z-neighbours stay in registers, x-neighbours come from
+// the adjacent lanes (DPP) with the tile's first and last interior lanes
+// loading the x halo, and each wave loads its y halo rows.  Accesses are
+// buffer instructions; lanes and rows outside the array (loads) or off the
+// interior (stores) carry an out-of-range offset instead of a branch.
 template <int RY, bool RHS>
 __device__ __forceinline__ void march_f64_tile(
     double *__restrict__ un, const double *__restrict__ uo,
     const double *__restrict__ rhs, const double *__restrict__ kap, int nx,
     int ny, int sy, long long sz, double factor, int k_start, int k_end,
-    int zchunk, int tx, int ty, int tz) {
+    int zchunk, int x0, int tx, int ty, int tz) {
 #pragma clang fp contract(off)
   // past every plane: planes hold at most 2^28 elements (2^31 bytes)
   constexpr unsigned OOB = 0xfffffff8u;
@@ -48,7 +51,7 @@ __device__ __forceinline__ void march_f64_tile(
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
 
-  const int i = 1 + tx * 64 + lane;
+  const int i = x0 + tx * 64 + lane;
   const int j0 = 1 + (ty * waves + wave) * RY;
   if (j0 > ny - 2) return;  // wave-uniform; no barriers in this function
   const int kb = k_start + tz * zchunk;
@@ -57,11 +60,12 @@ __device__ __forceinline__ void march_f64_tile(
   const int nz_last = k_end + 1;  // last plane any load of this range needs
   const unsigned b8 = (unsigned)((unsigned long long)sz * 8ull);
 
-  const bool lane_ld = i <= nx - 1;   // this lane's column exists
-  const bool lane_cmp = i <= nx - 2;  // and is interior
+  const bool lane_ld = i >= 0 && i <= nx - 1;   // this lane's column exists
+  const bool lane_cmp = i >= 1 && i <= nx - 2;  // and is interior
   // x halo: lane 0 holds the point left of the segment, lane 63 the point
-  // right of it (needed only when lane 63 itself is interior).
-  const bool xh_ld = (lane == 0) || (lane == 63 && lane_cmp);
+  // right of it (each needed only when that lane itself is interior; a
+  // lane left of it is then the boundary column, which lane 0 is not).
+  const bool xh_ld = (lane == 0 || lane == 63) && lane_cmp;
   const int xh_di = (lane == 0) ? -1 : 1;
 
   unsigned ro[RY], xo[RY], so[RY];

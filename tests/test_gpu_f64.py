@@ -71,6 +71,40 @@ def test_step_matches_oracle_bitwise(gpu, oracle, shape, tune):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("shape", [(64, 20, 12), (80, 33, 9), (144, 17, 10), (16, 16, 5),
+                                   (48, 70, 6)], ids=str)
+@pytest.mark.parametrize("shift", [0, 1, 2, 7, 15], ids=lambda s: f"shift{s}")
+@pytest.mark.parametrize("with_rhs", [False, True], ids=["norhs", "rhs"])
+def test_line_aligned_tile_origins(gpu, oracle, shape, shift, with_rhs):
+    """Dense rows a whole number of 128-byte lines long (nx a multiple of
+    16): the tiles' x origin follows the line offset of u_new, here a field
+    placed `shift` elements into its buffer, so lane 0 of a tile sits at
+    x = 0, at x = 1 or left of the array (x0 = 0, 1, -49, -50, -55 on a
+    line-aligned allocation).  Bit-identical to the oracle either way."""
+    C = gpu.C
+    nx, ny, nz = shape
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1)
+    rhs = np.random.default_rng(9).uniform(-1e-4, 1e-4, g.elems) if with_rhs else None
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    steps = 3
+    want = oracle_steps(oracle, g, u, k, factor, steps, rhs=rhs)
+    nbytes = g.elems * 8
+    bufs = [gpu.DeviceArray(nbytes + 256) for _ in range(4)]
+    pa, pb, pk, pr = (C.c_void_p(b.ptr.value + 8 * shift) for b in bufs)
+    for p, x in ((pa, u), (pb, u), (pk, k)) + (((pr, rhs),) if with_rhs else ()):
+        gpu.call("h3d_target_memcpy", p, gpu.ptr(x), C.c_size_t(nbytes), 0, -1)
+    s = gpu.Stream()
+    for i in range(steps):
+        src, dst = (pa, pb) if i % 2 == 0 else (pb, pa)
+        gpu.call("h3d_step_f64", dst, src, pr if with_rhs else None, pk, C.byref(g), factor,
+                 1, nz - 2, None, s.ptr)
+    s.sync()
+    got = np.empty(g.elems)
+    gpu.call("h3d_target_memcpy", gpu.ptr(got), pb if steps % 2 else pa, C.c_size_t(nbytes),
+             -1, 0)
+    assert np.array_equal(got, want)
+
+
 def test_pitched_layout_and_rhs(gpu, oracle):
     nx, ny, nz = 70, 21, 17
     g, u, k = seeded(gpu, nx, ny, nz, pitch=10, rows_pad=3)
