@@ -47,12 +47,13 @@ typedef struct h3d_grid {
  * return H3D_EARG):
  *   FP64 step and FP32 single-step pass: rows 1, 2, 4 (FP64 also 8),
  *     waves 1..16 for FP64, 1..8 for FP32;
- *   FP32 fused passes (kblock 2..6): rows 2 or 4, waves 4 or 8. */
+ *   FP32 fused passes (kblock 2..6) and FP64 fused passes (kblock 2..4):
+ *     rows 2 or 4, waves 4 or 8. */
 typedef struct h3d_tune {
   int rows;   /* y rows per lane held in registers                   */
   int waves;  /* wavefronts per workgroup, stacked in y               */
   int zchunk; /* planes marched by one workgroup                      */
-  int kblock; /* FP32 kernels: time steps fused per HBM pass, 1..6    */
+  int kblock; /* time steps fused per HBM pass: FP32 1..6, FP64 1..4  */
 } h3d_tune;
 
 /* ---------------------------------------------------------------------- */
@@ -93,10 +94,25 @@ int h3d_step_f64(double *u_new, const double *u_old, const double *rhs,
                  const double *kappa, const h3d_grid *g, double factor,
                  int k_start, int k_end, const h3d_tune *tune, void *stream);
 
+/* nsteps (1..4) FP64 updates of every interior point in one pass over HBM:
+ * u_out = nsteps steps of the reference's update applied to u_in, bit
+ * identical to nsteps calls of h3d_step_f64 (the same expression and
+ * association order, no contraction).  Planes are read once and written
+ * once per pass, 24/nsteps B per update (32/nsteps with rhs).  u_out's
+ * Dirichlet ring must hold u_in's.  No reference counterpart: the
+ * reference runs one kernel launch per step (1-openmp.c:193-205).  Shapes:
+ * rows 2 or 4, waves 4 or 8, any zchunk (H3D_ELIMIT when the shape's LDS
+ * exceeds 160 KB).  nsteps == 1 is h3d_step_f64 over all planes. */
+int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
+                 const double *kappa, const h3d_grid *g, double factor,
+                 int nsteps, const h3d_tune *tune, void *stream);
+
 /* `steps` FP64 updates with the u_a/u_b ping-pong of the reference's time
- * loop (1-openmp.c:193-205).  graph != 0 records two steps in a hipGraph
- * and replays it.  *ms receives the device time of the loop measured with
- * HIP events on `stream`; *final_in_b is 1 if the result is in u_b. */
+ * loop (1-openmp.c:193-205).  With tune->kblock = K > 1, every K steps are
+ * one h3d_pass_f64 (the remainder one shorter pass), and the buffers swap
+ * once per pass.  graph != 0 records two steps (passes) in a hipGraph and
+ * replays it.  *ms receives the device time of the loop measured with HIP
+ * events on `stream`; *final_in_b is 1 if the result is in u_b. */
 int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
                 const double *kappa, const h3d_grid *g, double factor,
                 int steps, const h3d_tune *tune, int graph, void *stream,
@@ -195,9 +211,11 @@ int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
 /* Choose tune fields for a problem by timing candidate shapes on the
  * device (a few passes each, reading u_a and kappa and writing a scratch
  * buffer the tuner allocates; u_b is not used); results are cached per
- * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step,
- * 1 = FP32 pass (kblock chosen too, 1..4, unless fixed_kblock > 0).  The
- * caller's buffers are not modified. */
+ * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step
+ * or pass (mode 0: no rhs, mode 1: with rhs, which the shapes' LDS must
+ * hold; kblock chosen too, 1..3, unless fixed_kblock > 0), 1 = FP32 pass
+ * (kblock chosen too, 1..4, unless fixed_kblock > 0).  The figure of merit
+ * is time per step.  The caller's buffers are not modified. */
 int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
                  int kappa_bf16, const h3d_grid *g, int mode,
                  int fixed_kblock, void *stream, h3d_tune *out);

@@ -86,6 +86,67 @@ int launch_march(double *un, const double *uo, const double *rhs,
   return h3d_launch_status();
 }
 
+// Temporally blocked FP64 passes: KB steps per HBM pass (pass_f64_tile;
+// the FP32 pass's tiling, h3d_f32.hip), one (tile, z chunk) per workgroup
+// with the chunk slowest and an XCD-aware order.
+template <int KB, int RY, int W, bool RHS>
+__global__ void __launch_bounds__(W * 64)
+h3d_pass_f64_kernel(double *__restrict__ uout, const double *__restrict__ uin,
+                    const double *__restrict__ rhs, const double *__restrict__ kf,
+                    int nx, int ny, int nz, int sy, long long sz, double factor,
+                    int zchunk, int tiles_x, int tiles_y) {
+  extern __shared__ double lds64[];
+  const unsigned b = h3d_xcd_tile(blockIdx.x, gridDim.x);
+  const int tx = (int)(b % (unsigned)tiles_x);
+  const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
+  const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
+  const int zb = 1 + tz * zchunk;
+  pass_f64_tile<KB, RY, W, RHS>(lds64, uout, uin, rhs, kf, nx, ny, nz, sy, sz, factor,
+                                zb, min(nz - 2, zb + zchunk - 1), tx, ty);
+}
+
+template <int KB, int RY, int W, bool RHS>
+int launch_pass_one(double *uo, const double *ui, const double *rhs, const double *kf,
+                    const h3d_grid *g, double factor, int zchunk, hipStream_t s) {
+  constexpr int OX = 64 - 2 * KB, OY = W * RY - 2 * KB;
+  if (OY < 1) return H3D_EARG;
+  constexpr size_t lds = sizeof(double) * (size_t)pass64_lds_doubles(KB, RY, W, RHS);
+  if (lds > 160 * 1024) return H3D_ELIMIT;
+  const int tiles_x = (g->nx - 2 + OX - 1) / OX;
+  const int tiles_y = (g->ny - 2 + OY - 1) / OY;
+  auto fn = h3d_pass_f64_kernel<KB, RY, W, RHS>;
+  if (lds > 64 * 1024) {
+    // dynamic LDS above 64 KB must be allowed per kernel, once
+    static bool big_lds_ok = false;
+    if (!big_lds_ok) {
+      H3D_TRY(hipFuncSetAttribute((const void *)fn,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024));
+      big_lds_ok = true;
+    }
+  }
+  const long long nb =
+      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+  if (nb > 0x7fffffffLL) return H3D_ELIMIT;
+  hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, s, uo, ui, rhs, kf,
+                     g->nx, g->ny, g->nz, g->sy, g->sz, factor, zchunk, tiles_x,
+                     tiles_y);
+  return h3d_launch_status();
+}
+
+template <int KB>
+int launch_pass_kb(int ry, int waves, double *uo, const double *ui, const double *rhs,
+                   const double *kf, const h3d_grid *g, double factor, int zchunk,
+                   hipStream_t s) {
+#define H3D_P(RY, W)                                                                   \
+  if (ry == RY && waves == W)                                                         \
+    return rhs ? launch_pass_one<KB, RY, W, true>(uo, ui, rhs, kf, g, factor, zchunk, s) \
+               : launch_pass_one<KB, RY, W, false>(uo, ui, rhs, kf, g, factor, zchunk, s);
+  H3D_P(2, 4) H3D_P(2, 8) H3D_P(4, 4) H3D_P(4, 8)
+#undef H3D_P
+  return H3D_EARG;
+}
+
 bool tune_ok(const h3d_tune &t) {
   if (!(t.rows == 1 || t.rows == 2 || t.rows == 4 || t.rows == 8)) return false;
   if (t.waves < 1 || t.waves > 16) return false;
@@ -136,6 +197,45 @@ extern "C" int h3d_step_f64(double *u_new, const double *u_old,
     case 2: return launch_march<2>(u_new, u_old, rhs, kappa, g, factor, k_start, k_end, t.waves, t.zchunk, s);
     case 4: return launch_march<4>(u_new, u_old, rhs, kappa, g, factor, k_start, k_end, t.waves, t.zchunk, s);
     default: return launch_march<8>(u_new, u_old, rhs, kappa, g, factor, k_start, k_end, t.waves, t.zchunk, s);
+  }
+}
+
+// Default shape of a fused pass: rows 2, waves 8, z chunks short enough for
+// 1024 workgroups (256 CUs x 4) while the z warm-up (2*kblock planes per
+// chunk) stays small.
+static void default_tune_pass64(const h3d_grid *g, int kb, h3d_tune *t) {
+  t->rows = 2;
+  t->waves = 8;
+  t->kblock = kb;
+  const int OX = 64 - 2 * kb, OY = t->rows * t->waves - 2 * kb;
+  const long long tiles = (long long)((g->nx - 2 + OX - 1) / OX) *
+                          ((g->ny - 2 + OY - 1) / OY);
+  const int planes = g->nz - 2;
+  int zc = planes;
+  while (zc > 8 * kb && tiles * ((planes + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
+  t->zchunk = zc;
+}
+
+extern "C" int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
+                            const double *kappa, const h3d_grid *g, double factor,
+                            int nsteps, const h3d_tune *tune, void *stream) {
+  if (!u_out || !u_in || !kappa || u_out == u_in || !h3d_grid_ok(g)) return H3D_EARG;
+  if (nsteps < 1 || nsteps > 4) return H3D_ELIMIT;
+  if (nsteps == 1)
+    return h3d_step_f64(u_out, u_in, rhs, kappa, g, factor, 1, g->nz - 2, tune, stream);
+  h3d_tune t;
+  default_tune_pass64(g, nsteps, &t);
+  if (tune) {
+    if (tune->rows) t.rows = tune->rows;
+    if (tune->waves) t.waves = tune->waves;
+    if (tune->zchunk) t.zchunk = tune->zchunk;
+  }
+  if (t.zchunk < 1) return H3D_EARG;
+  hipStream_t s = h3d_stream(stream);
+  switch (nsteps) {
+    case 2: return launch_pass_kb<2>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
+    case 3: return launch_pass_kb<3>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
+    default: return launch_pass_kb<4>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
   }
 }
 
@@ -201,14 +301,23 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
   if (!u_a || !u_b || !kappa || !h3d_grid_ok(g) || steps < 0) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
+  // kblock > 1: passes of kblock steps each (h3d_pass_f64), the remainder
+  // as one shorter pass; every pass swaps the buffers once
+  const int kb = (tune && tune->kblock > 1) ? tune->kblock : 1;
+  if (kb > 4) return H3D_ELIMIT;
+  auto pass = [&](double *out, const double *in, int ns) {
+    return ns == 1 ? h3d_step_f64(out, in, rhs, kappa, g, factor, 1, g->nz - 2, tune, s)
+                   : h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, tune, s);
+  };
+  const int full = steps / kb, rem = steps % kb;
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
   int rc = H3D_OK;
-  if (graph && steps >= 2) {
+  if (graph && full >= 2) {
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int r1 = h3d_step_f64(u_b, u_a, rhs, kappa, g, factor, 1, g->nz - 2, tune, s);
-    int r2 = h3d_step_f64(u_a, u_b, rhs, kappa, g, factor, 1, g->nz - 2, tune, s);
+    int r1 = pass(u_b, u_a, kb);
+    int r2 = pass(u_a, u_b, kb);
     hipError_t ec = hipStreamEndCapture(s, &R.gr);
     if (r1 != H3D_OK) return r1;
     if (r2 != H3D_OK) return r2;
@@ -218,15 +327,15 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
   H3D_TRY(hipEventRecord(R.e0, s));
   int done = 0;
   if (R.ge) {
-    for (; done + 2 <= steps; done += 2) {
+    for (; done + 2 <= full; done += 2) {
       hipError_t el = hipGraphLaunch(R.ge, s);
       if (el != hipSuccess) { rc = (int)el; break; }
     }
   }
-  for (; rc == H3D_OK && done < steps; ++done) {
+  const int npass = full + (rem ? 1 : 0);
+  for (; rc == H3D_OK && done < npass; ++done) {
     const bool even = (done % 2) == 0;
-    rc = h3d_step_f64(even ? u_b : u_a, even ? u_a : u_b, rhs, kappa, g,
-                      factor, 1, g->nz - 2, tune, s);
+    rc = pass(even ? u_b : u_a, even ? u_a : u_b, done < full ? kb : rem);
   }
   if (rc == H3D_OK) {
     H3D_TRY(hipEventRecord(R.e1, s));
@@ -234,7 +343,7 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
     float t = 0.f;
     H3D_TRY(hipEventElapsedTime(&t, R.e0, R.e1));
     if (ms) *ms = t;
-    if (final_in_b) *final_in_b = steps % 2;
+    if (final_in_b) *final_in_b = npass % 2;
   }
   return rc;
 }

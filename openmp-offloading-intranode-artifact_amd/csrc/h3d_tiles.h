@@ -481,6 +481,251 @@ __device__ __forceinline__ void pass_f32_tile(
   }
 }
 
+// ---- FP64 fused pass: KB steps of the reference's update per HBM pass ----
+//
+// The FP32 pass's tile and march (above) with the FP64 update of
+// march_f64_tile: every face term is the reference's expression with the
+// same operands (a flux computed for one cell is the opposite face's term
+// of its neighbour, bit for bit), u + factor*s or (u + rhs) + factor*s
+// without contraction, so KB fused steps give the bits of KB single steps.
+// With rhs, the rhs value of each cell joins its face coefficients in the
+// LDS ring (every level adds it).
+
+// LDS of one workgroup, in doubles:
+//   coef : [KB slots][NC: cx, cy, cz (, rhs)][TY rows][64]
+//   cye  : [KB+1][W][64]   cy of each wave's last row
+//   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
+__host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W, bool RHS) {
+  return KB * (RHS ? 4 : 3) * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+}
+
+template <int KB, int RY, int W, bool RHS>
+struct Pass64 {
+  static constexpr int TY = W * RY;
+  static constexpr int NC = RHS ? 4 : 3;
+  static constexpr int SLOT = NC * TY * 64;  // doubles per coefficient slot
+  // ---- per-lane state ----
+  double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
+  double FZ[KB + 1][RY];  // z flux above the last plane of level l
+  double kprev[2][RY];    // kappa rows of the previous loaded plane
+  double pu[2][RY];       // prefetched u rows (slot per phase)
+  double pk[2][RY + 1];   // prefetched kappa rows + the row above
+  double pr[2][RY];       // prefetched rhs rows (RHS)
+
+  // ---- geometry ----
+  const double *uin;
+  const double *kf;
+  const double *rh;
+  double *uout;
+  double factor;
+  int nz, w;
+  long long sz;
+  unsigned o8[RY + 1];  // byte offsets of rows j0..j0+RY in a plane
+  unsigned s8[RY];      // store offsets: o8, or OUT_OF_RANGE off the output region
+  bool lane_int;
+  bool row_int[RY];
+  int zb, ze;
+  double *coef;  // this lane, slot 0, cx, this wave's row 0
+  double *cye;   // this lane, slot 0, wave 0
+  double *ue;    // this lane, (phase 0, level 0, wave 0, lo)
+
+  __device__ __forceinline__ int slot(int p) const {  // p mod KB
+    const int s = p % KB;
+    return s < 0 ? s + KB : s;
+  }
+  __device__ __forceinline__ int eslot(int p) const {  // p mod KB+1
+    const int s = p % (KB + 1);
+    return s < 0 ? s + KB + 1 : s;
+  }
+  __device__ __forceinline__ static constexpr int ue_at(int ph, int l, int wave, int hi) {
+    return (((ph * KB + l) * W + wave) * 2 + hi) * 64;
+  }
+
+  __device__ __forceinline__ void load_plane(int p, int ph) {
+    const int pcl = p < 0 ? 0 : (p > nz - 1 ? nz - 1 : p);
+    const long long plane = (long long)pcl * sz;
+    const unsigned b8 = (unsigned)((unsigned long long)sz * 8ull);
+    const auto ru = plane_rsrc(uin + plane, b8);
+    const auto rk = plane_rsrc(kf + plane, b8);
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      pu[ph][r] = ld_f64(ru, o8[r]);
+      if (RHS) pr[ph][r] = ld_f64(plane_rsrc(rh + plane, b8), o8[r]);
+    }
+#pragma unroll
+    for (int r = 0; r <= RY; ++r) pk[ph][r] = ld_f64(rk, o8[r]);
+  }
+
+  // One march iteration at phase P (P alternates; slot roles swap).
+  template <int P>
+  __device__ __forceinline__ void iter(int it) {
+#pragma clang fp contract(off)
+    // 1. Face coefficients: cz of plane it-1 now (read by level 1 below),
+    //    cx/cy (and rhs) of plane it at the end (their slot is read below
+    //    as it-KB).
+    double cxn[RY], cyn[RY], rn[RY];
+    {
+      double *czs = coef + slot(it - 1) * SLOT + 2 * TY * 64;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const double k = pk[P][r];
+        cxn[r] = h3d_from_right(k, 0.0) + k;
+        cyn[r] = pk[P][r + 1] + k;
+        czs[r * 64] = kprev[1 - P][r] + k;
+        kprev[P][r] = k;
+        rn[r] = RHS ? pr[P][r] : 0.0;
+      }
+      cye[(eslot(it) * W + w) * 64] = cyn[RY - 1];
+    }
+    // 2. Level 0: the loaded plane becomes the newest slot of level 0.
+#pragma unroll
+    for (int r = 0; r < RY; ++r) A[0][1 - P][r] = pu[P][r];
+    // 3. Prefetch the next plane (clamped, so always a valid address).
+    load_plane(it + 1, 1 - P);
+
+    // 4. Levels 1..KB at planes it-1 .. it-KB.
+    const int wb = w > 0 ? w - 1 : 0;
+    const int wa = w < W - 1 ? w + 1 : W - 1;
+    const unsigned b8 = (unsigned)((unsigned long long)sz * 8ull);
+#pragma unroll
+    for (int l = 1; l <= KB; ++l) {
+      const int q = it - l;
+      const bool q_int = q >= 1 && q <= nz - 2;
+      const bool q_out = (l == KB) && q >= zb && q <= ze;
+      const double ub = ue[ue_at(1 - P, l - 1, wb, 1)];  // row below (mid plane)
+      const double ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
+      const double cyb = cye[(eslot(q) * W + wb) * 64];
+      const double *cs = coef + slot(q) * SLOT;
+      const long long qplane = (long long)(q_out ? q : zb) * sz;
+      const auto ro = plane_rsrc(uout + qplane, b8);
+      double fy_prev = 0.0;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        const double u = A[l - 1][P][r];
+        const double un = A[l - 1][1 - P][r];
+        const double ccx = cs[r * 64];
+        const double ccy = cs[TY * 64 + r * 64];
+        const double ccz = cs[2 * TY * 64 + r * 64];
+        const double fxp = ccx * (h3d_from_right(u, 0.0) - u);
+        const double fxm = h3d_from_left(fxp, 0.0);
+        const double ujp = (r < RY - 1) ? A[l - 1][P][r + 1] : ua;
+        const double fyp = ccy * (ujp - u);
+        const double fym = (r > 0) ? fy_prev : cyb * (u - ub);
+        fy_prev = fyp;
+        const double fzp = ccz * (un - u);
+        const double fzm = FZ[l][r];
+        FZ[l][r] = fzp;
+        const double s = ((((fxp - fxm) + fyp) - fym) + fzp) - fzm;
+        const double res = RHS ? (u + cs[3 * TY * 64 + r * 64]) + factor * s
+                               : u + factor * s;
+        if (l < KB) {
+          // Dirichlet cells keep their value
+          const bool inside = lane_int && row_int[r] && q_int;
+          A[l][1 - P][r] = inside ? res : u;
+        } else if (q_out) {
+          // stored cells are interior on every axis; rows and lanes off the
+          // output region carry an out-of-range offset (dropped write); the
+          // branch is uniform over the workgroup
+          st_f64(ro, s8[r], res);
+        }
+      }
+    }
+    // 5. Coefficients of plane it into the slot just vacated by it-KB, and
+    //    this iteration's newest edge rows of levels 0..KB-1.
+    {
+      double *cs = coef + slot(it) * SLOT;
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        cs[r * 64] = cxn[r];
+        cs[TY * 64 + r * 64] = cyn[r];
+        if (RHS) cs[3 * TY * 64 + r * 64] = rn[r];
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < KB; ++l) {
+      ue[ue_at(P, l, w, 0)] = A[l][1 - P][0];
+      ue[ue_at(P, l, w, 1)] = A[l][1 - P][RY - 1];
+    }
+    __syncthreads();
+  }
+};
+
+// One workgroup's tile (tx, ty), output planes zb..ze (1 <= zb <= ze <=
+// nz-2), of an FP64 pass of KB steps with W waves; `lds` holds
+// pass64_lds_doubles(KB, RY, W, RHS) doubles.  Every thread of the
+// workgroup must call it (it synchronises the workgroup).
+template <int KB, int RY, int W, bool RHS>
+__device__ __forceinline__ void pass_f64_tile(
+    double *lds, double *__restrict__ uout, const double *__restrict__ uin,
+    const double *__restrict__ rhs, const double *__restrict__ kf, int nx,
+    int ny, int nz, int sy, long long sz, double factor, int zb, int ze,
+    int tx, int ty) {
+  using S = Pass64<KB, RY, W, RHS>;
+  S st;
+  st.uin = uin; st.kf = kf; st.rh = rhs; st.uout = uout; st.factor = factor;
+  st.nz = nz; st.sz = sz;
+  const int lane = threadIdx.x & 63;
+  st.w = threadIdx.x >> 6;
+  constexpr int OX = 64 - 2 * KB, OY = S::TY - 2 * KB;
+
+  const int i = 1 - KB + tx * OX + lane;
+  const int y0 = 1 - KB + ty * OY;  // global y of tile row 0
+  const int j0 = y0 + st.w * RY;
+  st.zb = zb;
+  st.ze = ze;
+
+  // Loads are clamped into the array: cells outside it, and the Dirichlet
+  // ring, are carried through every level unchanged, and no interior cell
+  // reads anything but interior or ring cells, so clamped values never
+  // reach a written result.
+  const int ic = i < 0 ? 0 : (i > nx - 1 ? nx - 1 : i);
+  st.lane_int = i >= 1 && i <= nx - 2;
+  const bool lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
+#pragma unroll
+  for (int r = 0; r <= RY; ++r) {
+    const int j = j0 + r;
+    const int jc = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
+    st.o8[r] = (unsigned)(jc * sy + ic) * 8u;
+  }
+  // Planes are at most 2^28 elements (2 GiB of doubles), so 2^31 bytes is
+  // at or past the range of every plane resource.
+  constexpr unsigned OUT_OF_RANGE = 0x80000000u;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    const int j = j0 + r, tr = st.w * RY + r;
+    st.row_int[r] = j >= 1 && j <= ny - 2;
+    const bool out = lane_out && tr >= KB && tr <= S::TY - 1 - KB && j <= ny - 2;
+    st.s8[r] = out ? st.o8[r] : OUT_OF_RANGE;
+  }
+  st.coef = lds + st.w * RY * 64 + lane;
+  st.cye = lds + KB * S::SLOT + lane;
+  st.ue = lds + KB * S::SLOT + (KB + 1) * W * 64 + lane;
+
+#pragma unroll
+  for (int l = 0; l <= KB; ++l)
+#pragma unroll
+    for (int r = 0; r < RY; ++r) st.FZ[l][r] = 0.0;
+#pragma unroll
+  for (int r = 0; r < RY; ++r) {
+    st.kprev[0][r] = st.kprev[1][r] = 0.0;
+#pragma unroll
+    for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = 0.0;
+  }
+  // All LDS starts defined (warm-up iterations read slots not yet written).
+  for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W, RHS); x += W * 64) lds[x] = 0.0;
+  __syncthreads();
+
+  // Iterations it0 .. it0 + 2*pairs - 1 cover zb-KB .. ze+KB; an extra
+  // trailing iteration (odd count) writes nothing (its plane is > ze).
+  const int it0 = st.zb - KB;
+  const int pairs = (st.ze + KB - it0 + 2) / 2;
+  st.load_plane(it0, 0);
+  for (int p = 0; p < pairs; ++p) {
+    st.template iter<0>(it0 + 2 * p);
+    st.template iter<1>(it0 + 2 * p + 1);
+  }
+}
+
 // One tile (tx, ty, tz) of a single-step FP32 pass (KB = 1), built like the
 // FP64 tile: waves x 64 lanes, RY rows per lane, independent wavefronts (no
 // LDS, no barrier), z-neighbours in registers, x-neighbours through DPP with

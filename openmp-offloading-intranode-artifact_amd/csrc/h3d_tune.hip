@@ -2,9 +2,11 @@
 // pass on the caller's problem and keeps the fastest per problem shape.
 //
 // Candidates are timed with HIP events on the caller's stream, writing
-// into scratch output buffers (the caller's fields are only read), best
-// of three launches after one warm-up.  For the FP32 pass the figure of
-// merit is time per time step (pass time / kblock).
+// into scratch output buffers (the caller's fields are only read): one
+// warm-up launch, then the best of three samples of two back-to-back
+// launches each (single sub-millisecond launches were noisy enough to
+// swap shapes 8% apart).  The figure of merit is time per time step
+// (pass time / kblock).
 #include <mutex>
 #include <vector>
 
@@ -38,12 +40,13 @@ int time_best(hipStream_t s, F launch, float *best_ms) {
     float ms = 0;
     hipError_t e = hipEventRecord(e0, s);
     if (e == hipSuccess) rc = launch();
+    if (rc == H3D_OK && e == hipSuccess) rc = launch();
     if (rc != H3D_OK) break;
     if (e == hipSuccess) e = hipEventRecord(e1, s);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e != hipSuccess) { rc = (int)e; break; }
-    if (ms < best) best = ms;
+    if (ms / 2 < best) best = ms / 2;
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -60,9 +63,10 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   if (!u_a || !kappa || !out || !h3d_grid_ok(g)) return H3D_EARG;
   if (kind != 0 && kind != 1) return H3D_EARG;
   if (kind == 1 && (mode < 0 || mode > 2)) return H3D_EARG;
-  if (fixed_kblock < 0 || fixed_kblock > 6) return H3D_ELIMIT;
-  const Key key{kind, g->nx, g->ny, g->nz, g->sy, kind ? mode : 0,
-                kind ? kappa_bf16 : 0, kind ? fixed_kblock : 0, g->sz};
+  if (kind == 0 && (mode < 0 || mode > 1)) return H3D_EARG;
+  if (fixed_kblock < 0 || fixed_kblock > (kind ? 6 : 4)) return H3D_ELIMIT;
+  const Key key{kind, g->nx, g->ny, g->nz, g->sy, mode,
+                kind ? kappa_bf16 : 0, fixed_kblock, g->sz};
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     for (auto &e : cache)
@@ -84,31 +88,34 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   int rc = (e == hipSuccess) ? H3D_OK : (int)e;
   if (rc == H3D_OK && kind == 0) {
     const int planes = g->nz - 2;
-    // whole z range first: on large grids the chunk-edge re-reads cost more
-    // than the extra parallelism of short chunks buys
-    const int zcs[] = {planes, 256, 128, 64, 32};
-    for (int rows : {1, 2, 4})
-      for (int waves : {2, 4, 8})
-        for (int zc : zcs) {
-          if (zc > planes) continue;
-          h3d_tune t{rows, waves, zc, 1};
-          float ms;
-          int r = time_best(s, [&] {
-            return h3d_step_f64((double *)scratch, (const double *)u_a, nullptr,
-                                (const double *)kappa, g, 1e-3, 1, g->nz - 2,
-                                &t, s);
-          }, &ms);
-          if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
-          if (r == H3D_OK && ms < best_ms) {
-            best_ms = ms;
-            best = t;
-            if (best.zchunk == 0) {
-              h3d_tune d;
-              h3d_default_tune_f64(g, 1, g->nz - 2, &d);
-              best.zchunk = d.zchunk;
+    // mode 1 times with an rhs field (u_a stands in: only read)
+    const double *rhs = mode == 1 ? (const double *)u_a : nullptr;
+    for (int kbk = 1; kbk <= 3 && rc == H3D_OK; ++kbk) {
+      if (fixed_kblock && kbk != fixed_kblock) continue;
+      // single steps: whole z range first (on large grids the chunk-edge
+      // re-reads cost more than the extra parallelism of short chunks
+      // buys); fused passes: the LDS-shared tiles with short z chunks
+      const std::vector<int> rws = kbk == 1 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4};
+      const std::vector<int> wvs = kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8};
+      const std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
+                                            : std::vector<int>{32, 64, 128};
+      for (int rows : rws)
+        for (int waves : wvs)
+          for (int zc : zcs) {
+            if (zc > planes && zc != zcs.front()) continue;
+            h3d_tune t{rows, waves, zc, kbk};
+            float ms;
+            int r = time_best(s, [&] {
+              return h3d_pass_f64((double *)scratch, (const double *)u_a, rhs,
+                                  (const double *)kappa, g, 1e-3, kbk, &t, s);
+            }, &ms);
+            if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
+            if (r == H3D_OK && ms / kbk < best_ms) {
+              best_ms = ms / kbk;
+              best = t;
             }
           }
-        }
+    }
   } else if (rc == H3D_OK) {
     const float *kf = kappa_bf16 ? nullptr : (const float *)kappa;
     const uint16_t *kb = kappa_bf16 ? (const uint16_t *)kappa : nullptr;

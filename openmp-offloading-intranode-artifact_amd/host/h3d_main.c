@@ -22,7 +22,8 @@
  *                      FP64 plain (default); FP32 storage plain; FP32
  *                      compensated with register carry; with FP16 residual
  *   --kappa-bf16       store kappa in BF16 with a per-field scale (FP32)
- *   --kblock K         FP32: time steps fused per HBM pass (1..6)
+ *   --kblock K         time steps fused per HBM pass (FP32 1..6, FP64 1..4;
+ *                      the results are the same bits as single steps)
  *   --tile R,W,Z       rows per lane, waves per workgroup, planes per chunk
  *   --autotune         time candidate launch shapes first and use the best
  *                      (the default when there are at least 128^3 interior
@@ -169,6 +170,10 @@ static opts parse(int argc, char **argv)
   }
   if (o.slab_parts && o.prec != 0) {
     fprintf(stderr, "ERROR: --slab runs the FP64 kernel only\n");
+    exit(1);
+  }
+  if (o.slab_parts && o.tune.kblock > 1) {
+    fprintf(stderr, "ERROR: --slab runs single steps (no --kblock)\n");
     exit(1);
   }
   return o;
@@ -406,8 +411,11 @@ int main(int argc, char **argv)
   }
   if (o.autotune) {
     h3d_tune t;
-    CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &gd, mode,
-                       f32 ? o.tune.kblock : 0, stream, &t));
+    /* FP64: mode 1 tunes shapes that hold rhs; the slab step runs single
+     * steps */
+    CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &gd,
+                       f32 ? mode : (rhs ? 1 : 0),
+                       o.slab_parts ? 1 : o.tune.kblock, stream, &t));
     tune = t;
     printf("Autotuned: rows=%d waves=%d zchunk=%d kblock=%d\n", tune.rows,
            tune.waves, tune.zchunk, tune.kblock);
@@ -431,8 +439,9 @@ int main(int argc, char **argv)
     CHECK(h3d_run_slab_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, d_face_lo,
                            d_face_hi, steps, &tune, o.graph, stream, &dev_ms,
                            &in_b));
-  } else if (!f32 && o.graph) {
-    CHECK(h3d_run_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, steps, &tune, 1,
+  } else if (!f32 && (o.graph || tune.kblock > 1)) {
+    /* replayed from a hipGraph, or passes of tune.kblock fused steps */
+    CHECK(h3d_run_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, steps, &tune, o.graph,
                       stream, &dev_ms, &in_b));
   } else if (!f32) {
     /* the reference's loop (1-openmp.c:193-205): one update, then swap */
@@ -457,7 +466,7 @@ int main(int argc, char **argv)
   const double lups = (double)(n - 2) * (n - 2) * (nz - 2) * steps;
   double bpl = f32 ? (o.bf16 ? 10.0 : 12.0) : (rhs ? 32.0 : 24.0);
   if (o.prec == 3) bpl += 4.0;
-  if (f32) bpl /= (tune.kblock > 0 ? tune.kblock : 1);
+  if (!o.slab_parts) bpl /= (tune.kblock > 0 ? tune.kblock : 1);
   if (timer > 0)
     printf("Performance: %.3f GLUP/s, %.1f GB/s algorithmic (%.2f B/LUP)\n",
            lups / timer / 1e9, lups * bpl / timer / 1e9, bpl);

@@ -40,6 +40,7 @@ def test_every_entry_point_rejects_bad_grids(h3d, L, bad):
     ms, fb = C.c_float(), C.c_int()
     assert L.h3d_step_f64(u, v, None, k, g, 0.1, 1, 1, None, None) == EARG
     assert L.h3d_run_f64(u, v, None, k, g, 0.1, 3, None, 0, None, C.byref(ms), C.byref(fb)) == EARG
+    assert L.h3d_pass_f64(u, v, None, k, g, 0.1, 2, None, None) == EARG
     assert L.h3d_pass_f32(u, v, k, None, None, None, g, C.c_float(0.1), 1, 0, None, None) == EARG
     assert L.h3d_run_f32(u, v, k, None, None, None, g, C.c_float(0.1), 3, 0, None, 0, None,
                          C.byref(ms), C.byref(fb)) == EARG
@@ -69,6 +70,29 @@ def test_fp64_step_checks(h3d, L):
                          C.byref(fb)) == EARG
     assert L.h3d_run_f64(u, v, None, k, gp, 0.1, 4, None, 1, None, C.byref(ms),
                          C.byref(fb)) == EARG  # a graph needs a created stream
+
+
+def test_fp64_pass_checks(h3d, L):
+    g = grid(h3d)
+    gp = C.byref(g)
+    u, v, k, r = FAKE[:4]
+    P = L.h3d_pass_f64
+    assert P(u, u, None, k, gp, 0.1, 2, None, None) == EARG  # in place
+    assert P(None, v, None, k, gp, 0.1, 2, None, None) == EARG
+    assert P(u, v, None, None, gp, 0.1, 2, None, None) == EARG
+    assert P(u, v, None, k, gp, 0.1, 0, None, None) == ELIMIT
+    assert P(u, v, None, k, gp, 0.1, 5, None, None) == ELIMIT
+    for kblock, rows, waves in ((2, 1, 4), (2, 2, 2), (3, 8, 8), (4, 2, 4), (2, 2, 16)):
+        t = h3d.Tune(rows, waves, 8, kblock)
+        assert P(u, v, None, k, gp, 0.1, kblock, C.byref(t), None) == EARG
+    t = h3d.Tune(2, 8, -1, 2)
+    assert P(u, v, None, k, gp, 0.1, 2, C.byref(t), None) == EARG  # z chunk
+    t = h3d.Tune(4, 8, 8, 4)
+    assert P(u, v, r, k, gp, 0.1, 4, C.byref(t), None) == ELIMIT  # LDS above 160 KB
+    ms, fb = C.c_float(), C.c_int()
+    t = h3d.Tune(2, 8, 8, 5)
+    assert L.h3d_run_f64(u, v, None, k, gp, 0.1, 10, C.byref(t), 0, None, C.byref(ms),
+                         C.byref(fb)) == ELIMIT
 
 
 def test_fp32_pass_checks(h3d, L):
@@ -124,5 +148,7 @@ def test_ensemble_and_tuner_checks(h3d, L):
     assert L.h3d_autotune(2, u, v, k, 0, C.byref(g), 0, 0, None, C.byref(t)) == EARG
     assert L.h3d_autotune(1, u, v, k, 0, C.byref(g), 3, 0, None, C.byref(t)) == EARG
     assert L.h3d_autotune(1, u, v, k, 0, C.byref(g), 1, 7, None, C.byref(t)) == ELIMIT
+    assert L.h3d_autotune(0, u, v, k, 0, C.byref(g), 2, 0, None, C.byref(t)) == EARG  # FP64: rhs flag
+    assert L.h3d_autotune(0, u, v, k, 0, C.byref(g), 0, 5, None, C.byref(t)) == ELIMIT
     bad = grid(h3d, nx=2)
     assert L.h3d_autotune(0, u, v, k, 0, C.byref(bad), 0, 0, None, C.byref(t)) == EARG

@@ -244,3 +244,114 @@ def test_largest_plane_stride_bitwise(gpu, oracle):
     assert np.array_equal(got, want)
     for b in bufs:
         b.close()
+
+
+PASS_CASES = [
+    # shape, kblock, rows, waves, zchunk, rhs (shapes whose LDS fits 160 KB)
+    ((70, 40, 30), 2, 2, 8, 0, False),
+    ((70, 40, 30), 2, 2, 8, 0, True),
+    ((70, 40, 30), 2, 4, 4, 7, True),
+    ((65, 33, 21), 3, 2, 8, 5, False),
+    ((65, 33, 21), 3, 2, 8, 5, True),   # exactly 160 KB of LDS
+    ((130, 35, 19), 4, 4, 4, 0, False),
+    ((9, 75, 12), 2, 4, 8, 3, False),
+    ((9, 75, 12), 3, 4, 4, 3, True),
+    ((5, 5, 5), 2, 2, 4, 1, False),
+    ((5, 5, 5), 2, 2, 4, 1, True),
+    ((61, 61, 13), 3, 4, 4, 4, True),
+    ((66, 66, 66), 2, 4, 8, 16, False),
+    ((66, 66, 66), 2, 4, 4, 16, True),
+    ((40, 20, 50), 3, 2, 4, 6, False),  # two output rows per tile
+]
+
+
+@pytest.mark.parametrize("case", PASS_CASES, ids=str)
+def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case):
+    """Temporally blocked FP64 passes (h3d_run_f64 with kblock > 1): two
+    full passes and a remainder, bit-identical to the oracle stepping the
+    reference's update one step at a time."""
+    shape, kb, rows, waves, zc, with_rhs = case
+    nx, ny, nz = shape
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1)
+    rhs = np.random.default_rng(3).uniform(-1e-4, 1e-4, g.elems) if with_rhs else None
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    steps = 2 * kb + 1
+    want = oracle_steps(oracle, g, u, k, factor, steps, rhs=rhs)
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+    dr = gpu.DeviceArray.from_host(rhs) if with_rhs else None
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    t = gpu.Tune(rows, waves, zc, kb)
+    gpu.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr if dr else None, dk.ptr, gpu.C.byref(g),
+             factor, steps, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = (db if fb.value else da).download()
+    assert fb.value == 1  # three passes
+    assert np.array_equal(got, want)
+
+
+def test_fused_pass_pitched_and_graph(gpu, oracle):
+    """Padded rows and planes are never written; the hipGraph replay of
+    fused passes equals the eager loop."""
+    nx, ny, nz = 70, 21, 17
+    g, u, k = seeded(gpu, nx, ny, nz, pitch=10, rows_pad=3)
+    rhs = np.random.default_rng(5).uniform(-1e-4, 1e-4, g.elems)
+    factor = 0.02
+    steps = 9
+    want = oracle_steps(oracle, g, u, k, factor, steps, rhs=rhs)
+    t = gpu.Tune(2, 8, 5, 2)
+    s = gpu.Stream()
+    outs = []
+    for graph in (0, 1):
+        da, db, dk, dr = (gpu.DeviceArray.from_host(x) for x in (u, u, k, rhs))
+        ms, fb = gpu.C.c_float(), gpu.C.c_int()
+        gpu.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr, dk.ptr, gpu.C.byref(g), factor, steps,
+                 gpu.C.byref(t), graph, s.ptr, gpu.C.byref(ms), gpu.C.byref(fb))
+        outs.append((db if fb.value else da).download())
+    assert np.array_equal(outs[0], want) and np.array_equal(outs[1], want)
+    pad = outs[0].reshape(nz, ny + 3, nx + 10)
+    assert np.array_equal(pad[:, :, nx:], u.reshape(nz, ny + 3, nx + 10)[:, :, nx:])
+    assert np.array_equal(pad[:, ny:, :], u.reshape(nz, ny + 3, nx + 10)[:, ny:, :])
+
+
+@pytest.mark.parametrize("kb", [2, 3, 4])
+def test_fused_pass_equals_single_steps_258(gpu, kb):
+    """At 256^3 interior points (every z chunk, tile and XCD remap in use):
+    kb-step passes give the bits of single steps."""
+    n = 258
+    g = gpu.pitched_grid(n, n, n)
+    u, k = gpu.gen_seeded(g, 0, n, n, n, 0, 20260101)
+    factor = 0.9 * 2.0 / gpu.lib().h3d_face_sum_max_host(gpu.ptr(k), gpu.C.byref(g)) / 2.0
+    steps = 2 * kb
+    res = []
+    rows, waves = (4, 4) if kb == 4 else (2, 8)  # shapes whose LDS fits
+    for t in (gpu.Tune(2, 4, 0, 1), gpu.Tune(rows, waves, 32, kb)):
+        fs = []
+        for x in (u, u, k):
+            f = gpu.AlignedField(g, np.float64)
+            f.upload(x)
+            fs.append(f)
+        ms, fb = gpu.C.c_float(), gpu.C.c_int()
+        gpu.call("h3d_run_f64", fs[0].ptr, fs[1].ptr, None, fs[2].ptr, gpu.C.byref(g), factor,
+                 steps, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+        res.append((fs[1] if fb.value else fs[0]).download())
+    assert np.array_equal(res[0], res[1])
+
+
+def test_fused_pass_argument_errors(gpu):
+    g = gpu.Grid.dense(20)
+    d1, d2 = gpu.DeviceArray(g.elems * 8), gpu.DeviceArray(g.elems * 8)
+    L = gpu.lib()
+    f = 0.01
+    assert L.h3d_pass_f64(d1.ptr, d1.ptr, None, d2.ptr, gpu.C.byref(g), f, 2, None, None) == -1
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, None, gpu.C.byref(g), f, 2, None, None) == -1
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, None, None) == -2
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 0, None, None) == -2
+    t = gpu.Tune(2, 4, 0, 4)  # 8 rows cannot hold a 2*4 halo
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 4, gpu.C.byref(t),
+                          None) == -1
+    t = gpu.Tune(4, 8, 0, 4)  # 4 steps of 32 rows with rhs: LDS above 160 KB
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, d1.ptr, d2.ptr, gpu.C.byref(g), f, 4, gpu.C.byref(t),
+                          None) == -2
+    t = gpu.Tune(2, 8, 0, 5)
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    assert L.h3d_run_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, gpu.C.byref(t), 0,
+                         None, gpu.C.byref(ms), gpu.C.byref(fb)) == -2
