@@ -242,18 +242,20 @@ typedef struct h3d_ensemble h3d_ensemble;
 /* Builds the tile work lists on `device_id` and allocates the FP32
  * members' residual fields (zeroed).  Tiles are fixed in x and y (256-thread
  * workgroups).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
- * depth (planes per work item, default 64) and tune->kblock the FP32
+ * depth (planes per work item, default 64) and tune->kblock the
  * temporal-blocking depth (steps fused per pass of an FP32 member, 1..3,
- * default 1; H3D_ELIMIT above). */
+ * default 1; H3D_ELIMIT above; at 2, FP64 members fuse 2 steps per pass
+ * as well). */
 int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
                         int count, const h3d_tune *tune, int device_id);
-/* `steps` time steps of every member.  With kblock K, each group of K
- * steps is one launch (FP64 tiles one step, FP32 tiles a K-step pass) and
- * K-1 launches over the FP64 tiles; steps past the last full group are
+/* `steps` time steps of every member.  With kblock 2, each group of 2
+ * steps is one launch of 2-step passes for every member; with kblock 3,
+ * one launch (FP64 tiles one step, FP32 tiles a 3-step pass) and 2
+ * launches over the FP64 tiles; steps past the last full group are
  * single-step launches.  graph != 0 captures the run's launches into one
  * hipGraph (kept and reused while the step count and starting parities
- * repeat) and launches that.  *ms is the device time of the steps.  FP64
- * members ping-pong u_a -> u_b every step, FP32 members every pass:
+ * repeat) and launches that.  *ms is the device time of the steps.
+ * Members ping-pong u_a -> u_b once per launch that advances them:
  * h3d_ensemble_where says which buffer holds a member's current field. */
 int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph, void *stream,
                      float *ms);

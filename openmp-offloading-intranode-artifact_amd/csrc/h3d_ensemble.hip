@@ -14,12 +14,16 @@
 // persistent kernel the LDS tile measured 8% faster than the
 // wave-independent single-step tile, so it is used for kblock = 1 too.
 //
-// With kblock = K > 1 a group of K steps is one launch in which FP64 tiles
-// take one step and FP32 tiles a K-step pass, then K-1 launches over the
-// FP64 tiles only.  Steps beyond the last full group are single-step
-// launches over every tile.  So FP64 members flip their ping-pong buffers
-// every step and FP32 members once per pass (h3d_ensemble_where).  The
-// launches of a run are captured into one hipGraph on request.
+// With kblock = 2 a group of 2 steps is one launch in which every tile
+// takes a 2-step pass: FP64 tiles run the FP64 pass tile (4 rows per lane,
+// 4 waves), bit-identical to h3d_pass_f64.  With kblock = 3 a group is one
+// launch in which FP64 tiles take one step and FP32 tiles a 3-step pass,
+// then 2 launches over the FP64 tiles only (a 3-step FP64 pass of 4 waves
+// would leave 10 of its 16 rows, and its LDS one workgroup per CU).  Steps
+// beyond the last full group are single-step launches over every tile.
+// Every member flips its ping-pong buffers once per launch that advances
+// it (h3d_ensemble_where).  The launches of a run are captured into one
+// hipGraph on request.
 #include <vector>
 #include <algorithm>
 
@@ -32,6 +36,8 @@ constexpr int ENS_F64_ROWS = 2;
 constexpr int ENS_F32_RY = 4;
 constexpr int ENS_ZCHUNK = 64;  // default z-march depth
 constexpr int ENS_MAXKB = 3;    // deepest FP32 pass fused inside the ensemble
+constexpr int ENS_F64_FUSE = 2; // the depth at which FP64 members fuse too
+constexpr int ENS_F64_PRY = 4;  // rows per lane of the FP64 pass tile
 
 struct EMember {
   void *ua, *ub;
@@ -56,10 +62,18 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
     if (m.precision == 0) {
       const double *in = static_cast<const double *>(parity64 ? m.ub : m.ua);
       double *out = static_cast<double *>(parity64 ? m.ua : m.ub);
-      march_f64_tile<ENS_F64_ROWS, false>(out, in, nullptr,
-                                          static_cast<const double *>(m.kap),
-                                          m.nx, m.ny, m.sy, m.sz, m.factor, 1,
-                                          m.nz - 2, zchunk, 1, d.y, d.z, d.w);
+      if constexpr (KB == ENS_F64_FUSE) {
+        const int zb = 1 + d.w * zchunk;
+        pass_f64_tile<KB, ENS_F64_PRY, ENS_WAVES, false>(
+            reinterpret_cast<double *>(lds), out, in, nullptr,
+            static_cast<const double *>(m.kap), m.nx, m.ny, m.nz, m.sy, m.sz, m.factor, zb,
+            min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
+      } else {
+        march_f64_tile<ENS_F64_ROWS, false>(out, in, nullptr,
+                                            static_cast<const double *>(m.kap),
+                                            m.nx, m.ny, m.sy, m.sz, m.factor, 1,
+                                            m.nz - 2, zchunk, 1, d.y, d.z, d.w);
+      }
     } else {
       const float *in = static_cast<const float *>(parity32 ? m.ub : m.ua);
       float *out = static_cast<float *>(parity32 ? m.ua : m.ub);
@@ -71,14 +85,19 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
           m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
           min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
     }
-    // the next tile's FP32 set-up rewrites LDS
+    // the next tile's set-up rewrites LDS
     __syncthreads();
   }
 }
 
 template <int KB>
 constexpr size_t ens_lds() {
-  return sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
+  const size_t f32 = sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
+  const size_t f64 = KB == ENS_F64_FUSE
+                         ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY,
+                                                                       ENS_WAVES, false)
+                         : 0;
+  return f32 > f64 ? f32 : f64;
 }
 
 }  // namespace
@@ -90,9 +109,13 @@ struct h3d_ensemble {
   int kb;           // FP32 steps per pass
   long long done;   // steps taken so far
   long long passes; // FP32 passes taken so far
+  long long flips64; // FP64 buffer flips so far (steps, or passes at depth 2)
   EMember *d_mem;
-  struct Tiles { int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision; };
-  std::vector<Tiles> tiles;  // per member; [0] FP64 tiling, [d] FP32 at depth d
+  struct Tiles {
+    int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision;
+    int f64p_x, f64p_y;  // FP64 pass tiling at depth ENS_F64_FUSE
+  };
+  std::vector<Tiles> tiles;  // per member; [0] FP64 steps, [d] FP32 at depth d
   // Work lists in one device buffer: every tile at depth kb, every tile at
   // depth 1 (single-step launches), FP64 tiles only.
   int4 *d_work;
@@ -152,12 +175,14 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
     for (int m = 0; m < e->count; ++m) {
       const h3d_ensemble::Tiles &t = e->tiles[m];
       if (t.precision == 1 && !f32) continue;
-      const int tx_n = t.precision ? t.tiles_x[depth] : t.tiles_x[0];
-      const int ty_n = t.precision ? t.tiles_y[depth] : t.tiles_y[0];
+      const bool fused64 = !t.precision && depth == ENS_F64_FUSE;
+      const int tx_n = t.precision ? t.tiles_x[depth] : fused64 ? t.f64p_x : t.tiles_x[0];
+      const int ty_n = t.precision ? t.tiles_y[depth] : fused64 ? t.f64p_y : t.tiles_y[0];
       const int chunks = (t.planes + zchunk - 1) / zchunk;
       for (int tz = 0; tz < chunks; ++tz) {
         const int planes = std::min(zchunk, t.planes - tz * zchunk);
-        const long long cost = (long long)planes * (t.precision ? 2 * depth : 3);
+        // relative cost: HBM bytes per cell, 8 per plane and field
+        const long long cost = (long long)planes * (t.precision ? 2 * depth : fused64 ? 4 : 3);
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
       }
@@ -170,7 +195,9 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   };
   std::vector<int4> wkb = list(kb, true);
   std::vector<int4> w1 = kb > 1 ? list(1, true) : std::vector<int4>();
-  std::vector<int4> w64 = kb > 1 ? list(1, false) : std::vector<int4>();
+  // FP64-only single steps between fused FP32 passes (FP64 tiles fuse at
+  // depth ENS_F64_FUSE themselves)
+  std::vector<int4> w64 = kb > 1 && kb != ENS_F64_FUSE ? list(1, false) : std::vector<int4>();
   const size_t total = wkb.size() + w1.size() + w64.size();
   if (total > 0x7fffffffu) return H3D_ELIMIT;
   std::vector<int4> all;
@@ -234,19 +261,27 @@ static int ens_passes(const h3d_ensemble *e, int steps) {
   return full + (steps - full * e->kb);
 }
 
+// FP64 buffer flips in a run of `steps` steps: one per step, except that a
+// full group at depth ENS_F64_FUSE is one pass.
+static int ens_flips64(const h3d_ensemble *e, int steps) {
+  return e->kb == ENS_F64_FUSE ? ens_passes(e, steps) : steps;
+}
+
 // Issues the launches of `steps` steps from parities (p64, p32).
 static int ens_issue(h3d_ensemble *e, int steps, int p64, int p32, hipStream_t s) {
-  int rc = H3D_OK, np = 0, k = 0;
+  int rc = H3D_OK, np = 0, n64 = 0, k = 0;
   if (e->kb > 1) {
     for (; rc == H3D_OK && k + e->kb <= steps; k += e->kb) {
-      rc = ens_launch(e, 0, (p64 + k) % 2, (p32 + np) % 2, s);
+      rc = ens_launch(e, 0, (p64 + n64) % 2, (p32 + np) % 2, s);
       ++np;
-      for (int j = 1; rc == H3D_OK && j < e->kb; ++j)
-        rc = ens_launch(e, 2, (p64 + k + j) % 2, (p32 + np) % 2, s);
+      ++n64;
+      if (e->kb == ENS_F64_FUSE) continue;  // FP64 tiles took the whole group
+      for (int j = 1; rc == H3D_OK && j < e->kb; ++j, ++n64)
+        rc = ens_launch(e, 2, (p64 + n64) % 2, (p32 + np) % 2, s);
     }
   }
-  for (; rc == H3D_OK && k < steps; ++k, ++np)
-    rc = ens_launch(e, 1, (p64 + k) % 2, (p32 + np) % 2, s);
+  for (; rc == H3D_OK && k < steps; ++k, ++np, ++n64)
+    rc = ens_launch(e, 1, (p64 + n64) % 2, (p32 + np) % 2, s);
   return rc;
 }
 
@@ -292,6 +327,11 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
       t.tiles_x[d] = (x.g.nx - 2 + OX - 1) / OX;
       t.tiles_y[d] = (x.g.ny - 2 + OY - 1) / OY;
     }
+    {
+      const int OX = 64 - 2 * ENS_F64_FUSE, OY = ENS_F64_PRY * ENS_WAVES - 2 * ENS_F64_FUSE;
+      t.f64p_x = (x.g.nx - 2 + OX - 1) / OX;
+      t.f64p_y = (x.g.ny - 2 + OY - 1) / OY;
+    }
     if (x.precision == 1) {
       // residual fields start at zero; they live as long as the ensemble
       const size_t bytes = h3d_grid_elems(&x.g) * sizeof(unsigned short);
@@ -325,7 +365,7 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
   H3D_TRY(hipSetDevice(e->device));
-  const int p64 = (int)(e->done % 2), p32 = (int)(e->passes % 2);
+  const int p64 = (int)(e->flips64 % 2), p32 = (int)(e->passes % 2);
   if (graph && steps > 0 &&
       (e->graph_steps != steps || e->graph_p64 != p64 || e->graph_p32 != p32 || !e->graph)) {
     drop_graph(e);
@@ -360,11 +400,12 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
   if (ms) *ms = t;
   e->done += steps;
   e->passes += ens_passes(e, steps);
+  e->flips64 += ens_flips64(e, steps);
   return H3D_OK;
 }
 
-// Times candidate z-march depths and, when the ensemble has FP32 members,
-// FP32 pass depths 1..3, and keeps the fastest per step.  A depth-K group
+// Times candidate z-march depths and pass depths 1..3, and keeps the
+// fastest per step.  A depth-K group
 // costs one launch over every tile plus K-1 launches over the FP64 tiles;
 // each is timed on its own with the parities of the ensemble's next step,
 // so it reads the current fields and rewrites only the idle buffers with
@@ -373,9 +414,7 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
   if (!e) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   H3D_TRY(hipSetDevice(e->device));
-  const int p64 = (int)(e->done % 2), p32 = (int)(e->passes % 2);
-  bool has32 = false;
-  for (const auto &t : e->tiles) has32 = has32 || t.precision == 1;
+  const int p64 = (int)(e->flips64 % 2), p32 = (int)(e->passes % 2);
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
@@ -397,7 +436,8 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
   };
   int best_zc = e->zchunk, best_kb = e->kb, rc = H3D_OK;
   float best = 1e30f;
-  for (int kb = 1; kb <= (has32 ? ENS_MAXKB : 1) && rc == H3D_OK; ++kb)
+  // every depth is a candidate: FP64-only ensembles gain from depth 2 too
+  for (int kb = 1; kb <= ENS_MAXKB && rc == H3D_OK; ++kb)
     for (int zc : {32, 64, 128, 256}) {
       rc = build_work(e, zc, kb);
       if (rc != H3D_OK) break;
@@ -422,7 +462,7 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
 
 extern "C" int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b) {
   if (!e || !in_b || member < 0 || member >= e->count) return H3D_EARG;
-  const long long flips = e->tiles[member].precision ? e->passes : e->done;
+  const long long flips = e->tiles[member].precision ? e->passes : e->flips64;
   *in_b = (int)(flips % 2);
   return H3D_OK;
 }

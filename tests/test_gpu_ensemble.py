@@ -106,9 +106,11 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
     run(gpu, e, 3, 1, s)
     out = gpu.Tune()
     gpu.call("h3d_ensemble_autotune", e, s.ptr, C.byref(out))
-    assert out.zchunk in (32, 64, 128, 256)
+    assert out.zchunk in (32, 64, 128, 256) and out.kblock in (1, 2, 3)
     run(gpu, e, 2, 1, s)
     steps = 5
+    # passes: three single steps, then two steps at the tuner's depth
+    depths = [1, 1, 1] + pass_depths(2, out.kblock)
     for m, (sp, (da, db, dk)) in enumerate(zip(specs, keep)):
         got = (db if where(gpu, e, m) else da).download()
         g = sp["g"]
@@ -118,19 +120,22 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
             assert np.array_equal(got, want), g.nx
         else:
             f32 = float(np.float32(sp["factor"]))
-            want, _ = oracle.run_f32(sp["u"], sp["k"], None, g.nx, g.ny, g.nz, f32, steps,
-                                     1, 2)
-            assert got.tobytes() == want.tobytes(), g.nx
+            u, comp = sp["u"], None
+            for d in depths:
+                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
+                                         comp0=comp)
+            assert got.tobytes() == u.tobytes(), (g.nx, out.kblock)
     gpu.call("h3d_ensemble_destroy", e)
 
 
 @pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("kb", [2, 3])
 def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
-    """FP32 members fused kb steps per pass (FP64 members stepping alone in
-    between): each FP32 member equals the oracle replaying the same passes
-    with the FP16 residual carried across them, and each FP64 member its
-    plain run, over runs whose lengths are not multiples of kb."""
+    """Members fused kb steps per pass (FP64 members fused too at kb = 2,
+    stepping alone between FP32 passes at kb = 3): each FP32 member equals
+    the oracle replaying the same passes with the FP16 residual carried
+    across them, and each FP64 member its plain run, over runs whose
+    lengths are not multiples of kb."""
     specs = [make_member(gpu, d, m, 13, p) for m, (d, p) in enumerate(SPECS)]
     mems, keep = [], []
     for sp in specs:
@@ -151,7 +156,8 @@ def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
         got = (db if where(gpu, e, m) else da).download()
         g = sp["g"]
         if sp["precision"] == 0:
-            assert where(gpu, e, m) == steps % 2
+            # FP64 tiles fuse at depth 2 (one flip per pass), else step alone
+            assert where(gpu, e, m) == (len(depths) if kb == 2 else steps) % 2
             want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
                                   g.nz, steps, factor=sp["factor"])
             assert np.array_equal(got, want), g.nx
