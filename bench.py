@@ -68,6 +68,8 @@ def parse():
                     help="skip the slab, 1024^3 FP32 and ensemble configurations")
     ap.add_argument("--tune64", default="",
                     help="kblock,rows,waves,zchunk: skip FP64 autotuning")
+    ap.add_argument("--tune64s", default="",
+                    help="rows,waves,zchunk of the single-step FP64 leg: skip its autotuning")
     ap.add_argument("--tune32", default="",
                     help="kblock,rows,waves,zchunk: skip FP32 autotuning")
     return ap.parse_args()
@@ -486,7 +488,11 @@ def main():
     log(f"FP64: {a.steps} steps in {wall:.3f} s -> {glups:.1f} GLUP/s, "
         f"{achieved:.0f} GB/s ({100 * achieved / bw:.1f}% of stream)")
     # the single-step kernel (one launch per step, as the reference runs)
-    tune1 = tune if kb64 == 1 else tuned(1)
+    if a.tune64s:
+        r_, w_, z_ = (int(x) for x in a.tune64s.split(","))
+        tune1 = h3d.Tune(r_, w_, z_, 1)
+    else:
+        tune1 = tune if kb64 == 1 else tuned(1)
     glups1, wall1, per_launch1 = fp64_run(tune1)
     single = {"value": round(glups1, 3), "unit": "GLUP/s", "kblock": 1,
               "tune": {"rows": tune1.rows, "waves": tune1.waves, "zchunk": tune1.zchunk},
