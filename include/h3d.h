@@ -209,8 +209,9 @@ int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
                 int *final_in_b);
 
 /* Choose tune fields for a problem by timing candidate shapes on the
- * device (a few passes each, reading u_a and kappa and writing a scratch
- * buffer the tuner allocates; u_b is not used); results are cached per
+ * device (a few passes each, reading u_a and kappa and ping-ponging
+ * through two scratch fields the tuner allocates and seeds from u_a; u_b
+ * is not used); results are cached per
  * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step
  * or pass (mode 0: no rhs, mode 1: with rhs, which the shapes' LDS must
  * hold; kblock chosen too, 1..3, unless fixed_kblock > 0), 1 = FP32 pass
