@@ -39,6 +39,21 @@ def test_cli_reference_mode_matches_golden(entry, tmp_path):
     assert (tmp_path / "u_h3d_1gpu.txt").read_text() == want
 
 
+@pytest.mark.parametrize("kblock", [2, 3])
+@pytest.mark.parametrize("entry", [e for e in INDEX if e["n"] in (9, 17, 33)],
+                         ids=lambda e: f"n{e['n']}")
+def test_cli_fused_fp64_passes_match_golden(entry, kblock, tmp_path):
+    """`--kblock` fuses FP64 steps into passes (the reference fields carry
+    rhs): the dump is still the reference's serial program's, byte for
+    byte, including step counts that are not multiples of kblock."""
+    out = run([str(entry["n"]), entry["T"], "--kblock", str(kblock), "--no-autotune"],
+              str(tmp_path))
+    assert re.search(rf"^{entry['steps']} timesteps complete$", out, re.M)
+    with gzip.open(os.path.join(GOLDEN, entry["file"]), "rt") as f:
+        want = f.read()
+    assert (tmp_path / "u_h3d_1gpu.txt").read_text() == want
+
+
 def test_cli_graph_equals_eager(tmp_path):
     a = tmp_path / "a"
     b = tmp_path / "b"
