@@ -238,6 +238,11 @@ static int ens_launch(h3d_ensemble *e, int list, int p64, int p32, hipStream_t s
   if (n == 0) return H3D_OK;
   const int depth = list == 0 ? e->kb : 1;
   const int blocks = list == 0 ? e->blocks_kb : e->blocks_1;
+  if (depth == 2 && ens_lds<2>() > 64 * 1024) {
+    static std::atomic<unsigned long long> big_lds_done{0};
+    const int big_rc = h3d_allow_big_lds((const void *)ensemble_step<2>, &big_lds_done);
+    if (big_rc != H3D_OK) return big_rc;
+  }
   switch (depth) {
     case 1:
       hipLaunchKernelGGL(ensemble_step<1>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<1>(),

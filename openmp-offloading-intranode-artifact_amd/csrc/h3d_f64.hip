@@ -116,14 +116,9 @@ int launch_pass_one(double *uo, const double *ui, const double *rhs, const doubl
   const int tiles_y = (g->ny - 2 + OY - 1) / OY;
   auto fn = h3d_pass_f64_kernel<KB, RY, W, RHS>;
   if (lds > 64 * 1024) {
-    // dynamic LDS above 64 KB must be allowed per kernel, once
-    static bool big_lds_ok = false;
-    if (!big_lds_ok) {
-      H3D_TRY(hipFuncSetAttribute((const void *)fn,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  160 * 1024));
-      big_lds_ok = true;
-    }
+    static std::atomic<unsigned long long> big_lds_done{0};
+    const int big_rc = h3d_allow_big_lds((const void *)fn, &big_lds_done);
+    if (big_rc != H3D_OK) return big_rc;
   }
   const long long nb =
       (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
+
 #include "h3d.h"
 
 #define H3D_TRY(expr)                                  \
@@ -50,6 +52,19 @@ struct h3d_run_res {
     if (e1) (void)hipEventDestroy(e1);
   }
 };
+
+// Lets kernel `fn` use up to the CU's 160 KB of LDS as dynamic shared
+// memory (above the 64 KB default), once per kernel and device: bit d of
+// *done records device d (devices past 63 set the attribute every call).
+static inline int h3d_allow_big_lds(const void *fn, std::atomic<unsigned long long> *done) {
+  int dev = 0;
+  H3D_TRY(hipGetDevice(&dev));
+  const unsigned long long bit = dev < 64 ? 1ull << dev : 0ull;
+  if (bit && (done->load(std::memory_order_relaxed) & bit)) return H3D_OK;
+  H3D_TRY(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+  done->fetch_or(bit, std::memory_order_relaxed);
+  return H3D_OK;
+}
 
 static inline size_t h3d_grid_elems(const h3d_grid *g) {
   return (size_t)g->sz * (size_t)g->nz;
