@@ -22,13 +22,25 @@ def big(gpu):
     return dict(g=g, u=u, k=k, factor=dt / h / h / 2.0)
 
 
-def run64(gpu, P, u0, k, steps):
+def run64(gpu, P, u0, k, steps, tune=None):
     g = P["g"]
     da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u0, u0, k))
     ms, fb = gpu.C.c_float(), gpu.C.c_int()
     gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), P["factor"],
-             steps, None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+             steps, gpu.C.byref(tune) if tune else None, 0, None, gpu.C.byref(ms),
+             gpu.C.byref(fb))
     return (db, da) if fb.value else (da, db)
+
+
+@pytest.mark.parametrize("kb", [2, 3])
+def test_1000_steps_fused_passes_equal_single_steps(gpu, big, kb):
+    """The headline configuration as the benchmark runs it (fused FP64
+    passes, the tuner's 512^3 shapes) ends 1000 steps on the same bits as
+    1000 single steps."""
+    rows = 4 if kb == 2 else 2
+    a, _ = run64(gpu, big, big["u"], big["k"], 1000)
+    b, _ = run64(gpu, big, big["u"], big["k"], 1000, gpu.Tune(rows, 8, 128, kb))
+    assert a.download().tobytes() == b.download().tobytes()
 
 
 def test_full_field_bitwise_vs_oracle(gpu, oracle, big):
