@@ -488,22 +488,21 @@ __device__ __forceinline__ void pass_f32_tile(
 // same operands (a flux computed for one cell is the opposite face's term
 // of its neighbour, bit for bit), u + factor*s or (u + rhs) + factor*s
 // without contraction, so KB fused steps give the bits of KB single steps.
-// With rhs, the rhs value of each cell joins its face coefficients in the
-// LDS ring (every level adds it).
+// With rhs, each lane keeps the rhs of its cells for the last KB planes in
+// a register ring (every level adds it; a cell reads only its own).
 
-// LDS of one workgroup, in doubles:
-//   coef : [KB slots][NC: cx, cy, cz (, rhs)][TY rows][64]
+// LDS of one workgroup, in doubles (rhs needs none):
+//   coef : [KB slots][3: cx, cy, cz][TY rows][64]
 //   cye  : [KB+1][W][64]   cy of each wave's last row
 //   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
 __host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W, bool RHS) {
-  return KB * (RHS ? 4 : 3) * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+  return (void)RHS, KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
 }
 
 template <int KB, int RY, int W, bool RHS>
 struct Pass64 {
   static constexpr int TY = W * RY;
-  static constexpr int NC = RHS ? 4 : 3;
-  static constexpr int SLOT = NC * TY * 64;  // doubles per coefficient slot
+  static constexpr int SLOT = 3 * TY * 64;  // doubles per coefficient slot
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
@@ -511,6 +510,7 @@ struct Pass64 {
   double pu[2][RY];       // prefetched u rows (slot per phase)
   double pk[2][RY + 1];   // prefetched kappa rows + the row above
   double pr[2][RY];       // prefetched rhs rows (RHS)
+  double RH[KB][RY];      // rhs of planes it-1 .. it-KB (RHS; after the shift)
 
   // ---- geometry ----
   const double *uin;
@@ -573,7 +573,7 @@ struct Pass64 {
         cyn[r] = pk[P][r + 1] + k;
         czs[r * 64] = kprev[1 - P][r] + k;
         kprev[P][r] = k;
-        rn[r] = RHS ? pr[P][r] : 0.0;
+        rn[r] = RHS ? pr[P][r] : 0.0;  // rhs of plane it
       }
       cye[(eslot(it) * W + w) * 64] = cyn[RY - 1];
     }
@@ -616,8 +616,7 @@ struct Pass64 {
         const double fzm = FZ[l][r];
         FZ[l][r] = fzp;
         const double s = ((((fxp - fxm) + fyp) - fym) + fzp) - fzm;
-        const double res = RHS ? (u + cs[3 * TY * 64 + r * 64]) + factor * s
-                               : u + factor * s;
+        const double res = RHS ? (u + RH[l - 1][r]) + factor * s : u + factor * s;
         if (l < KB) {
           // Dirichlet cells keep their value
           const bool inside = lane_int && row_int[r] && q_int;
@@ -638,8 +637,16 @@ struct Pass64 {
       for (int r = 0; r < RY; ++r) {
         cs[r * 64] = cxn[r];
         cs[TY * 64 + r * 64] = cyn[r];
-        if (RHS) cs[3 * TY * 64 + r * 64] = rn[r];
       }
+    }
+    // rhs ring: plane it joins, plane it-KB leaves
+    if (RHS) {
+#pragma unroll
+      for (int l = KB - 1; l > 0; --l)
+#pragma unroll
+        for (int r = 0; r < RY; ++r) RH[l][r] = RH[l - 1][r];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) RH[0][r] = rn[r];
     }
 #pragma unroll
     for (int l = 0; l < KB; ++l) {
@@ -709,7 +716,7 @@ __device__ __forceinline__ void pass_f64_tile(
   for (int r = 0; r < RY; ++r) {
     st.kprev[0][r] = st.kprev[1][r] = 0.0;
 #pragma unroll
-    for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = 0.0;
+    for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = st.RH[l][r] = 0.0;
   }
   // All LDS starts defined (warm-up iterations read slots not yet written).
   for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W, RHS); x += W * 64) lds[x] = 0.0;
