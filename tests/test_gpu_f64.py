@@ -222,11 +222,14 @@ def get_rows(gpu, buf, dtype, nx, ny, nz, sy, sz):
     return out
 
 
-def test_largest_plane_stride_bitwise(gpu, oracle):
+@pytest.mark.parametrize("kb", [1, 2], ids=["step", "fused"])
+def test_largest_plane_stride_bitwise(gpu, oracle, kb):
     """Planes of 2^28 elements (2 GiB of FP64), the largest the kernels'
     32-bit in-plane byte offsets allow: a thin problem laid out with that
     plane stride steps bit-identically to the same values stored densely
-    (the last planes start 6 GiB into the buffers)."""
+    (the last planes start 6 GiB into the buffers), with single steps and
+    with a 2-step fused pass plus a single step (whose out-of-range store
+    offset, 2^31 bytes, equals such a plane's size)."""
     nx, ny, nz = 70, 5, 4
     sy, sz = 72, 1 << 28
     gd = gpu.Grid.dense(nx, ny, nz)
@@ -238,8 +241,10 @@ def test_largest_plane_stride_bitwise(gpu, oracle):
     for b, a in zip(bufs, (u, u, k)):
         put_rows(gpu, b, a, nx, ny, nz, sy, sz)
     ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    t = gpu.Tune(2, 8, 0, 2) if kb == 2 else None
     gpu.call("h3d_run_f64", bufs[0].ptr, bufs[1].ptr, None, bufs[2].ptr, gpu.C.byref(g),
-             factor, 3, None, 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+             factor, 3, gpu.C.byref(t) if t else None, 0, None, gpu.C.byref(ms),
+             gpu.C.byref(fb))
     got = get_rows(gpu, bufs[1] if fb.value else bufs[0], np.float64, nx, ny, nz, sy, sz)
     assert np.array_equal(got, want)
     for b in bufs:
