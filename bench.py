@@ -597,7 +597,10 @@ def main():
 
         # fastest kblock (the autotuner's choice, or --tune32 / --kblock)
         result["fp32c"] = fp32_leg(a.kblock, a.tune32)
-        # single-step passes: the stream-bound end of the same kernel family
+        # two fused steps: the temporally blocked depth closest to the stream
+        # rate; single-step passes: the stream-bound end of the family
+        if result["fp32c"]["kblock"] != 2:
+            result["fp32c_k2"] = fp32_leg(2)
         if result["fp32c"]["kblock"] != 1:
             result["fp32c_k1"] = fp32_leg(1)
 
