@@ -93,9 +93,8 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
 template <int KB>
 constexpr size_t ens_lds() {
   const size_t f32 = sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
-  const size_t f64 = KB == ENS_F64_FUSE
-                         ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY,
-                                                                       ENS_WAVES, false)
+  const size_t f64 =
+      KB == ENS_F64_FUSE ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY, ENS_WAVES)
                          : 0;
   return f32 > f64 ? f32 : f64;
 }

@@ -110,7 +110,7 @@ int launch_pass_one(double *uo, const double *ui, const double *rhs, const doubl
                     const h3d_grid *g, double factor, int zchunk, hipStream_t s) {
   constexpr int OX = 64 - 2 * KB, OY = W * RY - 2 * KB;
   if (OY < 1) return H3D_EARG;
-  constexpr size_t lds = sizeof(double) * (size_t)pass64_lds_doubles(KB, RY, W, RHS);
+  constexpr size_t lds = sizeof(double) * (size_t)pass64_lds_doubles(KB, RY, W);
   if (lds > 160 * 1024) return H3D_ELIMIT;
   const int tiles_x = (g->nx - 2 + OX - 1) / OX;
   const int tiles_y = (g->ny - 2 + OY - 1) / OY;

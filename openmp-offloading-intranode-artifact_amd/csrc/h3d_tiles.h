@@ -495,8 +495,8 @@ __device__ __forceinline__ void pass_f32_tile(
 //   coef : [KB slots][3: cx, cy, cz][TY rows][64]
 //   cye  : [KB+1][W][64]   cy of each wave's last row
 //   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
-__host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W, bool RHS) {
-  return (void)RHS, KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+__host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W) {
+  return KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
 }
 
 template <int KB, int RY, int W, bool RHS>
@@ -659,7 +659,7 @@ struct Pass64 {
 
 // One workgroup's tile (tx, ty), output planes zb..ze (1 <= zb <= ze <=
 // nz-2), of an FP64 pass of KB steps with W waves; `lds` holds
-// pass64_lds_doubles(KB, RY, W, RHS) doubles.  Every thread of the
+// pass64_lds_doubles(KB, RY, W) doubles.  Every thread of the
 // workgroup must call it (it synchronises the workgroup).
 template <int KB, int RY, int W, bool RHS>
 __device__ __forceinline__ void pass_f64_tile(
@@ -719,7 +719,7 @@ __device__ __forceinline__ void pass_f64_tile(
     for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = st.RH[l][r] = 0.0;
   }
   // All LDS starts defined (warm-up iterations read slots not yet written).
-  for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W, RHS); x += W * 64) lds[x] = 0.0;
+  for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W); x += W * 64) lds[x] = 0.0;
   __syncthreads();
 
   // Iterations it0 .. it0 + 2*pairs - 1 cover zb-KB .. ze+KB; an extra
