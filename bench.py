@@ -229,11 +229,54 @@ def leg_cfg3(bw, stream):
     st = h3d.stats(db if fb.value else da, g)
     for f in fields + [lo, hi]:
         f.close()
+    deep = {f"ghost{K}": slab_deep(n, planes - 2, gks[1], K, factor, steps, bw, stream)
+            for K in (2, 3)}
     return {"value": round(lups / (ms.value / 1e3) / 1e9, 3), "unit": "GLUP/s",
             "dtype": "f64", "steps": steps, "ms_per_step": round(ms.value / steps, 5),
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
             "slab": [n, n, planes], "nz_split": nzd, "face_bytes": n * n * 8,
             "tune": [t.rows, t.waves, t.zchunk],
+            "nonfinite": float(st[3]),
+            "fused_passes_deep_ghosts": deep}
+
+
+def slab_deep(n, owned, gk_owned, K, factor, steps, bw, stream):
+    """The same slab held with K ghost planes per side (n x n x (owned +
+    2K) points): fused passes of K steps, bands of K planes exchanged in
+    HBM once per pass, the interior overlapped with the bands.  Same update
+    count and factor as the one-ghost slab; the interior pass shape comes
+    from the autotuner at pass depth K."""
+    planes = owned + 2 * K
+    g = h3d.pitched_grid(n, n, planes)
+    u, k = h3d.gen_seeded(g, gk_owned - K, n, n, n, 1, 20260101)
+    fields = []
+    for x in (u, u, k):
+        f = h3d.AlignedField(g, np.float64)
+        f.upload(x)
+        fields.append(f)
+    da, db, dk = fields
+    lo, hi = h3d.DeviceArray(K * n * n * 8), h3d.DeviceArray(K * n * n * 8)
+    ms, fb = h3d.C.c_float(), h3d.C.c_int()
+    t = h3d.Tune()
+    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, K, stream.ptr,
+             h3d.C.byref(t))
+    args = (da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor, K, lo.ptr, hi.ptr)
+    h3d.call("h3d_run_slab_pass_f64", *args, 4 * K, h3d.C.byref(t), 1, stream.ptr,
+             h3d.C.byref(ms), h3d.C.byref(fb))
+    h3d.call("h3d_run_slab_pass_f64", *args, steps, h3d.C.byref(t), 1, stream.ptr,
+             h3d.C.byref(ms), h3d.C.byref(fb))
+    lups = (n - 2) * (n - 2) * owned * steps
+    passes = -(-steps // K)
+    # per pass: 2 band packs + 2 band unpacks of K planes, read and written
+    face_bytes = 4 * 2 * K * n * n * 8 * passes
+    gbs = (24.0 * lups / K + face_bytes) / (ms.value / 1e3) / 1e9
+    st = h3d.stats(db if fb.value else da, g)
+    for f in fields + [lo, hi]:
+        f.close()
+    return {"glups": round(lups / (ms.value / 1e3) / 1e9, 3),
+            "ms_per_step": round(ms.value / steps, 5), "achieved_gbs": round(gbs, 1),
+            "frac_of_stream": round(gbs / bw, 4), "slab": [n, n, planes],
+            "band_bytes": K * n * n * 8, "tune": [t.rows, t.waves, t.zchunk, t.kblock],
             "nonfinite": float(st[3])}
 
 

@@ -94,18 +94,22 @@ int h3d_step_f64(double *u_new, const double *u_old, const double *rhs,
                  const double *kappa, const h3d_grid *g, double factor,
                  int k_start, int k_end, const h3d_tune *tune, void *stream);
 
-/* nsteps (1..4) FP64 updates of every interior point in one pass over HBM:
- * u_out = nsteps steps of the reference's update applied to u_in, bit
- * identical to nsteps calls of h3d_step_f64 (the same expression and
- * association order, no contraction).  Planes are read once and written
- * once per pass, 24/nsteps B per update (32/nsteps with rhs).  u_out's
- * Dirichlet ring must hold u_in's.  No reference counterpart: the
- * reference runs one kernel launch per step (1-openmp.c:193-205).  Shapes:
- * rows 2 or 4, waves 4 or 8, any zchunk (H3D_ELIMIT when the shape's LDS
- * exceeds 160 KB).  nsteps == 1 is h3d_step_f64 over all planes. */
+/* nsteps (1..4) FP64 updates in one pass over HBM: planes k_start..k_end
+ * (1 <= k_start, k_end <= nz-2) of u_out receive those planes of nsteps
+ * steps of the reference's update applied to all of u_in, bit identical
+ * to nsteps calls of h3d_step_f64 over planes 1..nz-2 (the same expression
+ * and association order, no contraction).  An output plane depends on the
+ * input planes within nsteps of it; other planes of u_out are not written.
+ * Planes are read once and written once per pass, 24/nsteps B per update
+ * (32/nsteps with rhs).  u_out's Dirichlet ring must hold u_in's.  No
+ * reference counterpart: the reference runs one kernel launch per step
+ * (1-openmp.c:193-205).  Shapes: rows 2 or 4, waves 4 or 8, any zchunk
+ * (H3D_ELIMIT when the shape's LDS exceeds 160 KB).  nsteps == 1 is
+ * h3d_step_f64 over the same planes. */
 int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
                  const double *kappa, const h3d_grid *g, double factor,
-                 int nsteps, const h3d_tune *tune, void *stream);
+                 int nsteps, int k_start, int k_end, const h3d_tune *tune,
+                 void *stream);
 
 /* `steps` FP64 updates with the u_a/u_b ping-pong of the reference's time
  * loop (1-openmp.c:193-205).  With tune->kblock = K > 1, every K steps are
@@ -130,6 +134,14 @@ int h3d_pack_face_f32(float *buf, const float *u, const h3d_grid *g, int k,
                       void *stream);
 int h3d_unpack_face_f32(float *u, const float *buf, const h3d_grid *g, int k,
                         void *stream);
+/* Band pack/unpack: planes k .. k+count-1 to/from a dense [k][j][i] buffer
+ * of count*nx*ny values, the exchange unit of slabs with `count` ghost
+ * planes per side (h3d_slab_pass_f64).  No reference counterpart: the
+ * reference moves one plane per step (4-openmp-stream-omp-p2p.c:609-616). */
+int h3d_pack_band_f64(double *buf, const double *u, const h3d_grid *g, int k,
+                      int count, void *stream);
+int h3d_unpack_band_f64(double *u, const double *buf, const h3d_grid *g, int k,
+                        int count, void *stream);
 
 /* One step of one z-slab (nz >= 4 planes: 2 ghosts + owned planes) in the
  * shape of the reference's middle devices (4-openmp-stream-omp-p2p.c:
@@ -151,6 +163,37 @@ int h3d_run_slab_f64(double *u_a, double *u_b, const double *rhs,
                      double *face_lo, double *face_hi, int steps,
                      const h3d_tune *tune, int graph, void *stream, float *ms,
                      int *final_in_b);
+
+/* A slab with `ghost` (1..4) ghost planes per side: owned planes
+ * ghost .. nz-1-ghost, at least `ghost` of them (nz >= 3*ghost, nz >= 4).
+ * One pass advances the owned planes by nsteps (1..ghost) steps with one
+ * fused pass (h3d_pass_f64), the bands of `ghost` planes next to each side
+ * first, then pushes the bands into the ghosts of u_new through face_lo /
+ * face_hi (device buffers of ghost*nx*ny values): owned planes
+ * nz-2*ghost .. nz-1-ghost into ghosts 0 .. ghost-1 and owned planes
+ * ghost .. 2*ghost-1 into ghosts nz-ghost .. nz-1 (the z-periodic
+ * stand-in of h3d_slab_step_f64, `ghost` planes deep).  Between passes
+ * the ghosts must hold the neighbours' planes, as the exchange leaves
+ * them; kappa's and rhs's ghost planes hold the neighbours' values.  The
+ * same bytes cross per step as with one ghost plane, in 1/ghost of the
+ * messages.  ghost = 1, nsteps = 1 is h3d_slab_step_f64.  No reference
+ * counterpart: the reference exchanges one plane per step
+ * (4-openmp-stream-omp-p2p.c:609-616). */
+int h3d_slab_pass_f64(double *u_new, const double *u_old, const double *rhs,
+                      const double *kappa, const h3d_grid *g, double factor,
+                      int ghost, int nsteps, double *face_lo, double *face_hi,
+                      const h3d_tune *tune, void *stream);
+/* `steps` steps of such a slab as passes of `ghost` steps (the remainder
+ * one shorter pass) with u_a/u_b ping-pong, swapping once per pass; the
+ * owned planes between the bands run on a second stream; graph != 0
+ * replays a captured pair of passes.  tune shapes those interior passes
+ * (a fused-pass shape when ghost > 1).  stream must be a created stream.
+ * ghost = 1 is h3d_run_slab_f64. */
+int h3d_run_slab_pass_f64(double *u_a, double *u_b, const double *rhs,
+                          const double *kappa, const h3d_grid *g,
+                          double factor, int ghost, double *face_lo,
+                          double *face_hi, int steps, const h3d_tune *tune,
+                          int graph, void *stream, float *ms, int *final_in_b);
 
 /* FP64 diagnostics over the whole array (ring included), deterministic:
  * out[0] = sum u (checksum), out[1] = sum u^2, out[2] = max |u|,

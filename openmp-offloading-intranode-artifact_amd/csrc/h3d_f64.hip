@@ -94,20 +94,21 @@ __global__ void __launch_bounds__(W * 64)
 h3d_pass_f64_kernel(double *__restrict__ uout, const double *__restrict__ uin,
                     const double *__restrict__ rhs, const double *__restrict__ kf,
                     int nx, int ny, int nz, int sy, long long sz, double factor,
-                    int zchunk, int tiles_x, int tiles_y) {
+                    int ks, int ke, int zchunk, int tiles_x, int tiles_y) {
   extern __shared__ double lds64[];
   const unsigned b = h3d_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = (int)(b % (unsigned)tiles_x);
   const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
   const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
-  const int zb = 1 + tz * zchunk;
+  const int zb = ks + tz * zchunk;
   pass_f64_tile<KB, RY, W, RHS>(lds64, uout, uin, rhs, kf, nx, ny, nz, sy, sz, factor,
-                                zb, min(nz - 2, zb + zchunk - 1), tx, ty);
+                                zb, min(ke, zb + zchunk - 1), tx, ty);
 }
 
 template <int KB, int RY, int W, bool RHS>
 int launch_pass_one(double *uo, const double *ui, const double *rhs, const double *kf,
-                    const h3d_grid *g, double factor, int zchunk, hipStream_t s) {
+                    const h3d_grid *g, double factor, int ks, int ke, int zchunk,
+                    hipStream_t s) {
   constexpr int OX = 64 - 2 * KB, OY = W * RY - 2 * KB;
   if (OY < 1) return H3D_EARG;
   constexpr size_t lds = sizeof(double) * (size_t)pass64_lds_doubles(KB, RY, W);
@@ -121,22 +122,24 @@ int launch_pass_one(double *uo, const double *ui, const double *rhs, const doubl
     if (big_rc != H3D_OK) return big_rc;
   }
   const long long nb =
-      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+      (long long)tiles_x * tiles_y * ((ke - ks + 1 + zchunk - 1) / zchunk);
   if (nb > 0x7fffffffLL) return H3D_ELIMIT;
   hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, s, uo, ui, rhs, kf,
-                     g->nx, g->ny, g->nz, g->sy, g->sz, factor, zchunk, tiles_x,
-                     tiles_y);
+                     g->nx, g->ny, g->nz, g->sy, g->sz, factor, ks, ke, zchunk,
+                     tiles_x, tiles_y);
   return h3d_launch_status();
 }
 
 template <int KB>
 int launch_pass_kb(int ry, int waves, double *uo, const double *ui, const double *rhs,
-                   const double *kf, const h3d_grid *g, double factor, int zchunk,
-                   hipStream_t s) {
-#define H3D_P(RY, W)                                                                   \
-  if (ry == RY && waves == W)                                                         \
-    return rhs ? launch_pass_one<KB, RY, W, true>(uo, ui, rhs, kf, g, factor, zchunk, s) \
-               : launch_pass_one<KB, RY, W, false>(uo, ui, rhs, kf, g, factor, zchunk, s);
+                   const double *kf, const h3d_grid *g, double factor, int ks, int ke,
+                   int zchunk, hipStream_t s) {
+#define H3D_P(RY, W)                                                                 \
+  if (ry == RY && waves == W)                                                       \
+    return rhs ? launch_pass_one<KB, RY, W, true>(uo, ui, rhs, kf, g, factor, ks, ke, \
+                                                  zchunk, s)                        \
+               : launch_pass_one<KB, RY, W, false>(uo, ui, rhs, kf, g, factor, ks, ke, \
+                                                   zchunk, s);
   H3D_P(2, 4) H3D_P(2, 8) H3D_P(4, 4) H3D_P(4, 8)
 #undef H3D_P
   return H3D_EARG;
@@ -195,17 +198,17 @@ extern "C" int h3d_step_f64(double *u_new, const double *u_old,
   }
 }
 
-// Default shape of a fused pass: rows 2, waves 8, z chunks short enough for
-// 1024 workgroups (256 CUs x 4) while the z warm-up (2*kblock planes per
-// chunk) stays small.
-static void default_tune_pass64(const h3d_grid *g, int kb, h3d_tune *t) {
-  t->rows = 2;
-  t->waves = 8;
+// Default shape of a fused pass: rows 2, waves 8 (rows 4, waves 4 at 4
+// steps, where 2 x 8 needs 180 KB of LDS), z chunks short enough for 1024
+// workgroups (256 CUs x 4) while the z warm-up (2*kblock planes per chunk)
+// stays small.
+static void default_tune_pass64(const h3d_grid *g, int kb, int planes, h3d_tune *t) {
+  t->rows = kb == 4 ? 4 : 2;
+  t->waves = kb == 4 ? 4 : 8;
   t->kblock = kb;
   const int OX = 64 - 2 * kb, OY = t->rows * t->waves - 2 * kb;
   const long long tiles = (long long)((g->nx - 2 + OX - 1) / OX) *
                           ((g->ny - 2 + OY - 1) / OY);
-  const int planes = g->nz - 2;
   int zc = planes;
   while (zc > 8 * kb && tiles * ((planes + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
   t->zchunk = zc;
@@ -213,13 +216,16 @@ static void default_tune_pass64(const h3d_grid *g, int kb, h3d_tune *t) {
 
 extern "C" int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
                             const double *kappa, const h3d_grid *g, double factor,
-                            int nsteps, const h3d_tune *tune, void *stream) {
+                            int nsteps, int k_start, int k_end, const h3d_tune *tune,
+                            void *stream) {
   if (!u_out || !u_in || !kappa || u_out == u_in || !h3d_grid_ok(g)) return H3D_EARG;
+  if (k_start < 1 || k_end > g->nz - 2) return H3D_EARG;
   if (nsteps < 1 || nsteps > 4) return H3D_ELIMIT;
   if (nsteps == 1)
-    return h3d_step_f64(u_out, u_in, rhs, kappa, g, factor, 1, g->nz - 2, tune, stream);
+    return h3d_step_f64(u_out, u_in, rhs, kappa, g, factor, k_start, k_end, tune, stream);
+  if (k_end < k_start) return H3D_OK;  // empty range: nothing to do
   h3d_tune t;
-  default_tune_pass64(g, nsteps, &t);
+  default_tune_pass64(g, nsteps, k_end - k_start + 1, &t);
   if (tune) {
     if (tune->rows) t.rows = tune->rows;
     if (tune->waves) t.waves = tune->waves;
@@ -228,9 +234,9 @@ extern "C" int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs
   if (t.zchunk < 1) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   switch (nsteps) {
-    case 2: return launch_pass_kb<2>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
-    case 3: return launch_pass_kb<3>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
-    default: return launch_pass_kb<4>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, t.zchunk, s);
+    case 2: return launch_pass_kb<2>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, k_start, k_end, t.zchunk, s);
+    case 3: return launch_pass_kb<3>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, k_start, k_end, t.zchunk, s);
+    default: return launch_pass_kb<4>(t.rows, t.waves, u_out, u_in, rhs, kappa, g, factor, k_start, k_end, t.zchunk, s);
   }
 }
 
@@ -302,7 +308,7 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
   if (kb > 4) return H3D_ELIMIT;
   auto pass = [&](double *out, const double *in, int ns) {
     return ns == 1 ? h3d_step_f64(out, in, rhs, kappa, g, factor, 1, g->nz - 2, tune, s)
-                   : h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, tune, s);
+                   : h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, 1, g->nz - 2, tune, s);
   };
   const int full = steps / kb, rem = steps % kb;
   h3d_run_res R;

@@ -1,5 +1,7 @@
 // Face pack/unpack: one whole XY plane k (x/y ring included) to/from a
-// dense [j][i] buffer of nx*ny values.  This is the unit the reference's
+// dense [j][i] buffer of nx*ny values; band pack/unpack: `count`
+// consecutive planes to/from a dense [k][j][i] buffer (the exchange unit of
+// slabs that hold several ghost planes per side).  This is the unit the reference's
 // z-slab variants move between devices each step (2-openmp.c:321,
 // 4-openmp-stream-omp-p2p.c:609-616: an XY plane is contiguous in the
 // [k][j][i] layout, so the reference transfers it with one copy).  Here the
@@ -14,11 +16,14 @@ namespace {
 template <typename T>
 __global__ void __launch_bounds__(256)
 face_copy(T *__restrict__ dst, const T *__restrict__ src, int nx, int ny,
-          int dst_sy, int src_sy) {
-  // blockIdx.y walks rows, x threads walk a row.
+          int dst_sy, int src_sy, long long dst_sz, long long src_sz) {
+  // blockIdx.z is the plane of a band, blockIdx.y walks rows, x threads
+  // walk a row.
+  const T *sp = src + (long long)blockIdx.z * src_sz;
+  T *dp = dst + (long long)blockIdx.z * dst_sz;
   for (int j = blockIdx.y; j < ny; j += gridDim.y) {
-    const T *s = src + (long long)j * src_sy;
-    T *d = dst + (long long)j * dst_sy;
+    const T *s = sp + (long long)j * src_sy;
+    T *d = dp + (long long)j * dst_sy;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx;
          i += gridDim.x * blockDim.x)
       d[i] = s[i];
@@ -36,15 +41,19 @@ flat_copy(uint4 *__restrict__ dst, const uint4 *__restrict__ src, size_t nvec,
   if (t < (size_t)ntail) dtail[t] = stail[t];
 }
 
+// Planes k .. k+count-1 of u to/from a dense [k][j][i] buffer.
 template <typename T>
-int plane_copy(T *dst, const T *src, const h3d_grid *g, int k, bool pack,
+int plane_copy(T *dst, const T *src, const h3d_grid *g, int k, int count, bool pack,
                hipStream_t s) {
-  if (!dst || !src || !h3d_grid_ok(g) || k < 0 || k >= g->nz) return H3D_EARG;
+  if (!dst || !src || !h3d_grid_ok(g) || k < 0 || count < 1 || count > 65535 ||
+      k > g->nz - count)
+    return H3D_EARG;
   const long long plane = (long long)k * g->sz;
+  const long long face = (long long)g->nx * g->ny;
   T *d = pack ? dst : dst + plane;
   const T *sr = pack ? src + plane : src;
-  if (g->sy == g->nx) {
-    const size_t bytes = (size_t)g->nx * g->ny * sizeof(T);
+  if (g->sy == g->nx && (count == 1 || g->sz == face)) {
+    const size_t bytes = (size_t)face * count * sizeof(T);
     const bool aligned = ((uintptr_t)d % 16 == 0) && ((uintptr_t)sr % 16 == 0);
     if (aligned) {
       const size_t nvec = bytes / 16;
@@ -61,9 +70,11 @@ int plane_copy(T *dst, const T *src, const h3d_grid *g, int k, bool pack,
   }
   const int dsy = pack ? g->nx : g->sy;
   const int ssy = pack ? g->sy : g->nx;
-  dim3 grid((unsigned)((g->nx + 255) / 256), (unsigned)min(g->ny, 4096));
+  const long long dsz = pack ? face : g->sz;
+  const long long ssz = pack ? g->sz : face;
+  dim3 grid((unsigned)((g->nx + 255) / 256), (unsigned)min(g->ny, 4096), (unsigned)count);
   hipLaunchKernelGGL((face_copy<T>), grid, dim3(256), 0, s, d, sr, g->nx,
-                     g->ny, dsy, ssy);
+                     g->ny, dsy, ssy, dsz, ssz);
   return h3d_launch_status();
 }
 
@@ -71,17 +82,25 @@ int plane_copy(T *dst, const T *src, const h3d_grid *g, int k, bool pack,
 
 extern "C" int h3d_pack_face_f64(double *buf, const double *u,
                                  const h3d_grid *g, int k, void *stream) {
-  return plane_copy<double>(buf, u, g, k, true, h3d_stream(stream));
+  return plane_copy<double>(buf, u, g, k, 1, true, h3d_stream(stream));
 }
 extern "C" int h3d_unpack_face_f64(double *u, const double *buf,
                                    const h3d_grid *g, int k, void *stream) {
-  return plane_copy<double>(u, buf, g, k, false, h3d_stream(stream));
+  return plane_copy<double>(u, buf, g, k, 1, false, h3d_stream(stream));
 }
 extern "C" int h3d_pack_face_f32(float *buf, const float *u, const h3d_grid *g,
                                  int k, void *stream) {
-  return plane_copy<float>(buf, u, g, k, true, h3d_stream(stream));
+  return plane_copy<float>(buf, u, g, k, 1, true, h3d_stream(stream));
 }
 extern "C" int h3d_unpack_face_f32(float *u, const float *buf,
                                    const h3d_grid *g, int k, void *stream) {
-  return plane_copy<float>(u, buf, g, k, false, h3d_stream(stream));
+  return plane_copy<float>(u, buf, g, k, 1, false, h3d_stream(stream));
+}
+extern "C" int h3d_pack_band_f64(double *buf, const double *u, const h3d_grid *g,
+                                 int k, int count, void *stream) {
+  return plane_copy<double>(buf, u, g, k, count, true, h3d_stream(stream));
+}
+extern "C" int h3d_unpack_band_f64(double *u, const double *buf, const h3d_grid *g,
+                                   int k, int count, void *stream) {
+  return plane_copy<double>(u, buf, g, k, count, false, h3d_stream(stream));
 }

@@ -109,7 +109,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       const void *in = j == 0 ? u_a : scr[(j + 1) % 2];
       if (kind == 0)
         return h3d_pass_f64((double *)out, (const double *)in, rhs, (const double *)kappa,
-                            g, 1e-3, t.kblock, &t, s);
+                            g, 1e-3, t.kblock, 1, g->nz - 2, &t, s);
       return h3d_pass_f32((float *)out, (const float *)in, kf, kb16,
                           (uint16_t *)(j % 2 ? c_in : c_out),
                           (const uint16_t *)(j % 2 ? c_out : c_in), g, 1e-3f, t.kblock, mode,

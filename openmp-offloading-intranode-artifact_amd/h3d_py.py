@@ -61,12 +61,17 @@ SIGNATURES = {
     "h3d_heat_update_boundary_bottom": (_I, [_P, _P, _P, _P, _I, _I, _I, _D, _I]),
     "h3d_heat_update_boundary_top": (_I, [_P, _P, _P, _P, _I, _I, _I, _D, _I]),
     "h3d_step_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _I, _TP, _P]),
-    "h3d_pass_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _TP, _P]),
+    "h3d_pass_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _I, _I, _TP, _P]),
     "h3d_run_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _TP, _I, _P,
                          C.POINTER(C.c_float), C.POINTER(_I)]),
     "h3d_slab_step_f64": (_I, [_P, _P, _P, _P, _GP, _D, _P, _P, _TP, _P]),
     "h3d_run_slab_f64": (_I, [_P, _P, _P, _P, _GP, _D, _P, _P, _I, _TP, _I, _P,
                               C.POINTER(C.c_float), C.POINTER(_I)]),
+    "h3d_slab_pass_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _I, _P, _P, _TP, _P]),
+    "h3d_run_slab_pass_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _P, _P, _I, _TP, _I,
+                                   _P, C.POINTER(C.c_float), C.POINTER(_I)]),
+    "h3d_pack_band_f64": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_unpack_band_f64": (_I, [_P, _P, _GP, _I, _I, _P]),
     "h3d_pack_face_f64": (_I, [_P, _P, _GP, _I, _P]),
     "h3d_unpack_face_f64": (_I, [_P, _P, _GP, _I, _P]),
     "h3d_pack_face_f32": (_I, [_P, _P, _GP, _I, _P]),

@@ -40,7 +40,7 @@ def test_every_entry_point_rejects_bad_grids(h3d, L, bad):
     ms, fb = C.c_float(), C.c_int()
     assert L.h3d_step_f64(u, v, None, k, g, 0.1, 1, 1, None, None) == EARG
     assert L.h3d_run_f64(u, v, None, k, g, 0.1, 3, None, 0, None, C.byref(ms), C.byref(fb)) == EARG
-    assert L.h3d_pass_f64(u, v, None, k, g, 0.1, 2, None, None) == EARG
+    assert L.h3d_pass_f64(u, v, None, k, g, 0.1, 2, 1, 1, None, None) == EARG
     assert L.h3d_pass_f32(u, v, k, None, None, None, g, C.c_float(0.1), 1, 0, None, None) == EARG
     assert L.h3d_run_f32(u, v, k, None, None, None, g, C.c_float(0.1), 3, 0, None, 0, None,
                          C.byref(ms), C.byref(fb)) == EARG
@@ -50,6 +50,9 @@ def test_every_entry_point_rejects_bad_grids(h3d, L, bad):
     assert L.h3d_compare_f64(u, v, g, 1e-6, 1e-5, out, None) == EARG
     assert L.h3d_face_sum_max_f64(k, g, out, None) == EARG
     assert L.h3d_slab_step_f64(u, v, None, k, g, 0.1, f1, f2, None, FAKE[7]) == EARG
+    assert L.h3d_slab_pass_f64(u, v, None, k, g, 0.1, 1, 1, f1, f2, None, FAKE[7]) == EARG
+    assert L.h3d_pack_band_f64(f1, u, g, 0, 1, None) == EARG
+    assert L.h3d_unpack_band_f64(u, f1, g, 0, 1, None) == EARG
 
 
 def test_fp64_step_checks(h3d, L):
@@ -77,18 +80,20 @@ def test_fp64_pass_checks(h3d, L):
     gp = C.byref(g)
     u, v, k, r = FAKE[:4]
     P = L.h3d_pass_f64
-    assert P(u, u, None, k, gp, 0.1, 2, None, None) == EARG  # in place
-    assert P(None, v, None, k, gp, 0.1, 2, None, None) == EARG
-    assert P(u, v, None, None, gp, 0.1, 2, None, None) == EARG
-    assert P(u, v, None, k, gp, 0.1, 0, None, None) == ELIMIT
-    assert P(u, v, None, k, gp, 0.1, 5, None, None) == ELIMIT
+    assert P(u, u, None, k, gp, 0.1, 2, 1, 14, None, None) == EARG  # in place
+    assert P(None, v, None, k, gp, 0.1, 2, 1, 14, None, None) == EARG
+    assert P(u, v, None, None, gp, 0.1, 2, 1, 14, None, None) == EARG
+    assert P(u, v, None, k, gp, 0.1, 0, 1, 14, None, None) == ELIMIT
+    assert P(u, v, None, k, gp, 0.1, 5, 1, 14, None, None) == ELIMIT
     for kblock, rows, waves in ((2, 1, 4), (2, 2, 2), (3, 8, 8), (4, 2, 4), (2, 2, 16)):
         t = h3d.Tune(rows, waves, 8, kblock)
-        assert P(u, v, None, k, gp, 0.1, kblock, C.byref(t), None) == EARG
+        assert P(u, v, None, k, gp, 0.1, kblock, 1, 14, C.byref(t), None) == EARG
     t = h3d.Tune(2, 8, -1, 2)
-    assert P(u, v, None, k, gp, 0.1, 2, C.byref(t), None) == EARG  # z chunk
+    assert P(u, v, None, k, gp, 0.1, 2, 1, 14, C.byref(t), None) == EARG  # z chunk
     t = h3d.Tune(4, 8, 8, 4)
-    assert P(u, v, r, k, gp, 0.1, 4, C.byref(t), None) == ELIMIT  # LDS above 160 KB
+    assert P(u, v, r, k, gp, 0.1, 4, 1, 14, C.byref(t), None) == ELIMIT  # LDS above 160 KB
+    for ks, ke in ((0, 14), (1, 15), (-3, 2)):  # ring planes or outside the array
+        assert P(u, v, None, k, gp, 0.1, 2, ks, ke, None, None) == EARG
     ms, fb = C.c_float(), C.c_int()
     t = h3d.Tune(2, 8, 8, 5)
     assert L.h3d_run_f64(u, v, None, k, gp, 0.1, 10, C.byref(t), 0, None, C.byref(ms),
@@ -132,6 +137,22 @@ def test_face_and_slab_checks(h3d, L):
     ms, fb = C.c_float(), C.c_int()
     assert L.h3d_run_slab_f64(u, v, None, k, gp, 0.1, f1, f2, 4, None, 0, None,
                               C.byref(ms), C.byref(fb)) == EARG  # needs a created stream
+    # bands: planes k .. k+count-1 must lie in the array
+    for plane, count in ((-1, 2), (15, 2), (14, 3), (1, 0)):
+        assert L.h3d_pack_band_f64(f1, u, gp, plane, count, None) == EARG
+        assert L.h3d_unpack_band_f64(u, f1, gp, plane, count, None) == EARG
+    # deep slabs: ghost 1..4, nz >= 3*ghost, nsteps 1..ghost
+    SP, RP = L.h3d_slab_pass_f64, L.h3d_run_slab_pass_f64
+    for ghost, nsteps in ((0, 1), (5, 1), (2, 0), (2, 3)):
+        assert SP(u, v, None, k, gp, 0.1, ghost, nsteps, f1, f2, None, s) == EARG
+    g11 = grid(h3d, nz=11)
+    assert SP(u, v, None, k, C.byref(g11), 0.1, 4, 4, f1, f2, None, s) == EARG
+    assert SP(u, u, None, k, gp, 0.1, 2, 2, f1, f2, None, s) == EARG
+    assert SP(u, v, None, k, gp, 0.1, 2, 2, f1, f1, None, s) == EARG
+    assert RP(u, v, None, k, gp, 0.1, 2, f1, f2, -1, None, 0, s, C.byref(ms),
+              C.byref(fb)) == EARG
+    assert RP(u, v, None, k, gp, 0.1, 2, f1, f2, 4, None, 0, None, C.byref(ms),
+              C.byref(fb)) == EARG  # needs a created stream
 
 
 def test_ensemble_and_tuner_checks(h3d, L):

@@ -293,6 +293,44 @@ def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case):
     assert np.array_equal(got, want)
 
 
+RANGE_CASES = [
+    # shape, nsteps, k_start, k_end, rows, waves, zchunk, rhs, row pitch
+    ((40, 30, 24), 2, 5, 11, 2, 8, 0, False, 0),
+    ((40, 30, 24), 3, 1, 1, 2, 8, 0, True, 0),     # the bottom plane alone
+    ((40, 30, 24), 3, 22, 22, 2, 4, 0, False, 0),  # the top plane alone
+    ((70, 21, 30), 2, 3, 26, 4, 4, 5, True, 10),   # chunked, pitched
+    ((33, 40, 20), 4, 4, 15, 4, 4, 3, False, 0),
+    ((40, 30, 24), 1, 6, 9, 2, 2, 0, True, 0),     # one step: h3d_step_f64
+    ((33, 40, 20), 4, 1, 18, 0, 0, 0, True, 0),    # default shape at 4 steps
+]
+
+
+@pytest.mark.parametrize("case", RANGE_CASES, ids=str)
+def test_fused_pass_plane_range(gpu, oracle, case):
+    """A pass over planes k_start..k_end writes exactly those planes'
+    interior cells, with the bits of nsteps oracle steps over the whole
+    array; every other value of u_out (ring, padding, other planes) keeps
+    what it held."""
+    shape, ns, ks, ke, rows, waves, zc, with_rhs, pitch = case
+    nx, ny, nz = shape
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1, pitch=pitch)
+    rhs = np.random.default_rng(4).uniform(-1e-4, 1e-4, g.elems) if with_rhs else None
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz, g.sy, g.sz) / 2.0
+    full = oracle_steps(oracle, g, u, k, factor, ns, rhs=rhs)
+    pre = np.random.default_rng(9).standard_normal(g.elems)
+    want = pre.copy()
+    w3, f3 = (a.reshape(nz, ny, g.sy) for a in (want, full))
+    w3[ks:ke + 1, 1:ny - 1, 1:nx - 1] = f3[ks:ke + 1, 1:ny - 1, 1:nx - 1]
+    di, do, dk = (gpu.DeviceArray.from_host(x) for x in (u, pre, k))
+    dr = gpu.DeviceArray.from_host(rhs) if with_rhs else None
+    t = gpu.Tune(rows, waves, zc, ns)
+    s = gpu.Stream()
+    gpu.call("h3d_pass_f64", do.ptr, di.ptr, dr.ptr if dr else None, dk.ptr, gpu.C.byref(g),
+             factor, ns, ks, ke, gpu.C.byref(t), s.ptr)
+    s.sync()
+    assert np.array_equal(do.download(), want)
+
+
 def test_fused_pass_pitched_and_graph(gpu, oracle):
     """Padded rows and planes are never written; the hipGraph replay of
     fused passes equals the eager loop."""
@@ -346,15 +384,15 @@ def test_fused_pass_argument_errors(gpu):
     d1, d2 = gpu.DeviceArray(g.elems * 8), gpu.DeviceArray(g.elems * 8)
     L = gpu.lib()
     f = 0.01
-    assert L.h3d_pass_f64(d1.ptr, d1.ptr, None, d2.ptr, gpu.C.byref(g), f, 2, None, None) == -1
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, None, gpu.C.byref(g), f, 2, None, None) == -1
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, None, None) == -2
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 0, None, None) == -2
+    assert L.h3d_pass_f64(d1.ptr, d1.ptr, None, d2.ptr, gpu.C.byref(g), f, 2, 1, 18, None, None) == -1
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, None, gpu.C.byref(g), f, 2, 1, 18, None, None) == -1
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, 1, 18, None, None) == -2
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 0, 1, 18, None, None) == -2
     t = gpu.Tune(2, 4, 0, 4)  # 8 rows cannot hold a 2*4 halo
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 4, gpu.C.byref(t),
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 4, 1, 18, gpu.C.byref(t),
                           None) == -1
     t = gpu.Tune(4, 8, 0, 4)  # 4 steps of 32 rows with rhs: LDS above 160 KB
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, d1.ptr, d2.ptr, gpu.C.byref(g), f, 4, gpu.C.byref(t),
+    assert L.h3d_pass_f64(d1.ptr, d2.ptr, d1.ptr, d2.ptr, gpu.C.byref(g), f, 4, 1, 18, gpu.C.byref(t),
                           None) == -2
     t = gpu.Tune(2, 8, 0, 5)
     ms, fb = gpu.C.c_float(), gpu.C.c_int()

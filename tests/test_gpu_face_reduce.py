@@ -135,6 +135,153 @@ def test_slab_step_with_inhbm_exchange(gpu, oracle, graph, nz):
     assert np.array_equal(one.download(), want)
 
 
+@pytest.mark.parametrize("dims,pitch,k,count", [((7, 5, 9), 0, 2, 3), ((33, 17, 8), 5, 0, 8),
+                                                ((130, 66, 6), 0, 4, 2), ((40, 9, 7), 3, 6, 1)],
+                         ids=str)
+def test_band_pack_unpack_bitwise(gpu, dims, pitch, k, count):
+    """Planes k..k+count-1 to a dense [k][j][i] buffer and back, dense
+    (one flat copy) and pitched (row copies)."""
+    nx, ny, nz = dims
+    g = gpu.Grid(nx, ny, nz, nx + pitch, (nx + pitch) * ny)
+    rng = np.random.default_rng(nz * count)
+    u = rng.standard_normal(g.elems)
+    u.view(np.uint8)[::89] = 0xFF
+    du = gpu.DeviceArray.from_host(u)
+    buf = gpu.DeviceArray(count * nx * ny * 8)
+    s = gpu.Stream()
+    gpu.call("h3d_pack_band_f64", buf.ptr, du.ptr, gpu.C.byref(g), k, count, s.ptr)
+    s.sync()
+    u3 = u.reshape(nz, ny, nx + pitch)
+    assert buf.download().tobytes() == u3[k:k + count, :, :nx].ravel().tobytes()
+    band = rng.standard_normal(count * nx * ny)
+    buf.upload(band)
+    k2 = nz - count
+    gpu.call("h3d_unpack_band_f64", du.ptr, buf.ptr, gpu.C.byref(g), k2, count, s.ptr)
+    s.sync()
+    want = u.copy().reshape(nz, ny, nx + pitch)
+    want[k2:, :, :nx] = band.reshape(count, ny, nx)
+    assert du.download().tobytes() == want.ravel().tobytes()
+
+
+def deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, steps):
+    """Passes of `ghost` steps (the last one shorter) of the whole slab
+    array, each followed by the periodic band copies: owned planes
+    nz-2*ghost..nz-1-ghost into ghosts 0..ghost-1, owned planes
+    ghost..2*ghost-1 into ghosts nz-ghost..nz-1."""
+    P = n * n
+    a = u.copy()
+    left = steps
+    while left:
+        r = min(ghost, left)
+        a = oracle.run_f64(a.copy(), a.copy(), rhs, k, n, n, nz, r, factor=factor).copy()
+        a[0:ghost * P] = a[(nz - 2 * ghost) * P:(nz - ghost) * P]
+        a[(nz - ghost) * P:] = a[ghost * P:2 * ghost * P]
+        left -= r
+    return a
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+@pytest.mark.parametrize("ghost,nz", [(1, 5), (2, 6), (2, 9), (2, 23), (3, 9), (3, 16),
+                                      (4, 30)], ids=str)
+def test_deep_slab_standin_matches_oracle(gpu, oracle, graph, ghost, nz):
+    """h3d_run_slab_pass_f64 (fused passes of `ghost` steps, boundary bands
+    first, interior on a second stream, band exchange into the slab's own
+    ghosts) against the oracle stepping the whole slab array and copying
+    the bands; with ghost 1 it is h3d_run_slab_f64."""
+    n, steps = 37, 2 * ghost + 3
+    g = gpu.Grid.dense(n, n, nz)
+    u, k = gpu.gen_seeded(g, 0, n, n, nz, 1, 5)
+    u[:] = np.where(u == 0.0, 0.5, u)  # ghosts hold data, as in a real slab
+    rhs = np.random.default_rng(6).uniform(0, 1e-4, g.elems)
+    factor = 0.02
+    want = deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, steps)
+    da, db, dk, dr = (gpu.DeviceArray.from_host(x) for x in (u, u, k, rhs))
+    lo, hi = gpu.DeviceArray(ghost * n * n * 8), gpu.DeviceArray(ghost * n * n * 8)
+    s = gpu.Stream()
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    t = gpu.Tune(4, 4, 0, ghost) if ghost > 1 else None
+    gpu.call("h3d_run_slab_pass_f64", da.ptr, db.ptr, dr.ptr, dk.ptr, gpu.C.byref(g), factor,
+             ghost, lo.ptr, hi.ptr, steps, gpu.C.byref(t) if t else None, graph, s.ptr,
+             gpu.C.byref(ms), gpu.C.byref(fb))
+    npass = -(-steps // ghost)
+    assert fb.value == npass % 2
+    assert np.array_equal((db if fb.value else da).download(), want)
+    if ghost == 1:
+        ea, eb = gpu.DeviceArray.from_host(u), gpu.DeviceArray.from_host(u)
+        gpu.call("h3d_run_slab_f64", ea.ptr, eb.ptr, dr.ptr, dk.ptr, gpu.C.byref(g), factor,
+                 lo.ptr, hi.ptr, steps, None, graph, s.ptr, gpu.C.byref(ms), gpu.C.byref(fb))
+        assert np.array_equal((eb if fb.value else ea).download(), want)
+    # one shorter pass through the single-pass entry point
+    one, src = gpu.DeviceArray.from_host(u), gpu.DeviceArray.from_host(u)
+    r = max(1, ghost - 1)
+    gpu.call("h3d_slab_pass_f64", one.ptr, src.ptr, dr.ptr, dk.ptr, gpu.C.byref(g), factor,
+             ghost, r, lo.ptr, hi.ptr, None, s.ptr)
+    s.sync()
+    assert np.array_equal(one.download(),
+                          deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, r))
+
+
+@pytest.mark.parametrize("parts,ghost", [(2, 2), (4, 2), (4, 3), (2, 4), (4, 4)], ids=str)
+def test_deep_slabs_with_band_exchange_equal_global_run(gpu, oracle, parts, ghost):
+    """The deep-ghost algorithm across slabs: split a reference-mode
+    problem into the reference's z-slabs, widened to `ghost` ghost planes
+    per side (the first and last slab end at the global Dirichlet planes);
+    advance each slab by fused passes over its owned planes
+    (h3d_pass_f64 with a plane range) and swap bands of `ghost` planes
+    between neighbours (h3d_pack_band_f64 -> h3d_unpack_band_f64) once per
+    pass.  The owned planes reassembled equal the global run bit for bit."""
+    n, steps = 38, 2 * ghost + 1
+    h, dt, factor = gpu.params_reference(n)
+    u, rhs, k = gpu.init_reference(n)
+    glob = oracle.run_f64(u.copy(), u.copy(), rhs, k, n, n, n, steps, factor=factor)
+    rc, nzd, gks = gpu.slab_decompose(n, parts)
+    assert rc == 0
+    P = n * n
+    slabs = []
+    for nz_l, gk in zip(nzd, gks):
+        own_lo, own_hi = gk, gk + nz_l - 3  # owned global planes (2 ghosts in nz_l)
+        assert own_hi - own_lo + 1 >= ghost
+        start, end = max(0, own_lo - ghost), min(n - 1, own_hi + ghost)
+        sl = slice(start * P, (end + 1) * P)
+        slabs.append(dict(g=gpu.Grid.dense(n, n, end - start + 1), start=start,
+                          lo=own_lo - start, hi=own_hi - start,
+                          old=gpu.DeviceArray.from_host(u[sl]),
+                          new=gpu.DeviceArray.from_host(u[sl]),
+                          rhs=gpu.DeviceArray.from_host(rhs[sl]),
+                          kap=gpu.DeviceArray.from_host(k[sl])))
+    band = gpu.DeviceArray(ghost * P * 8)
+    s = gpu.Stream()
+    left = steps
+    while left:
+        r = min(ghost, left)
+        for sl in slabs:
+            gpu.call("h3d_pass_f64", sl["new"].ptr, sl["old"].ptr, sl["rhs"].ptr,
+                     sl["kap"].ptr, gpu.C.byref(sl["g"]), factor, r, sl["lo"], sl["hi"],
+                     None, s.ptr)
+        for p in range(parts - 1):
+            a, b = slabs[p], slabs[p + 1]
+            # top owned band of a -> lower ghosts of b; bottom owned band of
+            # b -> upper ghosts of a
+            gpu.call("h3d_pack_band_f64", band.ptr, a["new"].ptr, gpu.C.byref(a["g"]),
+                     a["hi"] - ghost + 1, ghost, s.ptr)
+            gpu.call("h3d_unpack_band_f64", b["new"].ptr, band.ptr, gpu.C.byref(b["g"]), 0,
+                     ghost, s.ptr)
+            gpu.call("h3d_pack_band_f64", band.ptr, b["new"].ptr, gpu.C.byref(b["g"]), b["lo"],
+                     ghost, s.ptr)
+            gpu.call("h3d_unpack_band_f64", a["new"].ptr, band.ptr, gpu.C.byref(a["g"]),
+                     a["hi"] + 1, ghost, s.ptr)
+        for sl in slabs:
+            sl["old"], sl["new"] = sl["new"], sl["old"]
+        left -= r
+    s.sync()
+    out = u.copy()  # global Dirichlet planes 0 and n-1 never change
+    for sl in slabs:
+        a = sl["old"].download()
+        g0 = sl["start"]
+        out[(g0 + sl["lo"]) * P:(g0 + sl["hi"] + 1) * P] = a[sl["lo"] * P:(sl["hi"] + 1) * P]
+    assert np.array_equal(out, glob)
+
+
 def test_stats_matches_oracle(gpu, oracle):
     n = 45
     g = gpu.Grid(n, n - 3, n - 5, n + 3, (n + 3) * (n - 3))
