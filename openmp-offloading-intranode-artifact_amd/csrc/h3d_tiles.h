@@ -490,25 +490,30 @@ __device__ __forceinline__ void pass_f32_tile(
 // without contraction, so KB fused steps give the bits of KB single steps.
 // With rhs, each lane keeps the rhs of its cells for the last KB planes in
 // a register ring (every level adds it; a cell reads only its own).
+//
+// kappa itself, not the face sums, sits in an LDS ring of KB+1 planes:
+// every sum k_a + k_b is formed where it is used, from the same two
+// operands, so it has the bits a stored sum would have.  Three sums per
+// cell and plane would take 3*KB planes of LDS; the kappa ring takes KB+1,
+// which lets two 8-wave workgroups of 32 rows share a CU at KB = 2 and fits
+// 32-row tiles at KB = 3 and 4.
 
 // LDS of one workgroup, in doubles (rhs needs none):
-//   coef : [KB slots][3: cx, cy, cz][TY rows][64]
-//   cye  : [KB+1][W][64]   cy of each wave's last row
-//   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
+//   kr : [KB+1 slots][TY rows][64]        kappa of planes it-KB .. it
+//   ue : [2][KB][W][2][64]  edge rows of every level, double buffered
 __host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W) {
-  return KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+  return (KB + 1) * (W * RY) * 64 + 2 * KB * W * 2 * 64;
 }
 
 template <int KB, int RY, int W, bool RHS>
 struct Pass64 {
   static constexpr int TY = W * RY;
-  static constexpr int SLOT = 3 * TY * 64;  // doubles per coefficient slot
+  static constexpr int SLOT = TY * 64;  // doubles per kappa slot
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
-  double kprev[2][RY];    // kappa rows of the previous loaded plane
   double pu[2][RY];       // prefetched u rows (slot per phase)
-  double pk[2][RY + 1];   // prefetched kappa rows + the row above
+  double pk[2][RY];       // prefetched kappa rows
   double pr[2][RY];       // prefetched rhs rows (RHS)
   double RH[KB][RY];      // rhs of planes it-1 .. it-KB (RHS; after the shift)
 
@@ -520,19 +525,16 @@ struct Pass64 {
   double factor;
   int nz, w;
   long long sz;
-  unsigned o8[RY + 1];  // byte offsets of rows j0..j0+RY in a plane
+  unsigned o8[RY];      // byte offsets of rows j0..j0+RY-1 in a plane
   unsigned s8[RY];      // store offsets: o8, or OUT_OF_RANGE off the output region
   bool lane_int;
   bool row_int[RY];
   int zb, ze;
-  double *coef;  // this lane, slot 0, cx, this wave's row 0
-  double *cye;   // this lane, slot 0, wave 0
-  double *ue;    // this lane, (phase 0, level 0, wave 0, lo)
+  double *kr;        // this lane, slot 0, this wave's row 0
+  const double *kb;  // this lane, slot 0, the row below this wave's rows
+  const double *ka;  // this lane, slot 0, the row above this wave's rows
+  double *ue;        // this lane, (phase 0, level 0, wave 0, lo)
 
-  __device__ __forceinline__ int slot(int p) const {  // p mod KB
-    const int s = p % KB;
-    return s < 0 ? s + KB : s;
-  }
   __device__ __forceinline__ int eslot(int p) const {  // p mod KB+1
     const int s = p % (KB + 1);
     return s < 0 ? s + KB + 1 : s;
@@ -551,35 +553,33 @@ struct Pass64 {
     for (int r = 0; r < RY; ++r) {
       pu[ph][r] = ld_f64(ru, o8[r]);
       if (RHS) pr[ph][r] = ld_f64(plane_rsrc(rh + plane, b8), o8[r]);
+      pk[ph][r] = ld_f64(rk, o8[r]);
     }
-#pragma unroll
-    for (int r = 0; r <= RY; ++r) pk[ph][r] = ld_f64(rk, o8[r]);
   }
 
   // One march iteration at phase P (P alternates; slot roles swap).
   template <int P>
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
-    // 1. Face coefficients: cz of plane it-1 now (read by level 1 below),
-    //    cx/cy (and rhs) of plane it at the end (their slot is read below
-    //    as it-KB).
-    double cxn[RY], cyn[RY], rn[RY];
+    // 1. kappa of plane it into its ring slot (the slot of it-KB-1, which
+    //    no level reads any more).  Other waves read these rows from the
+    //    next iteration on, after the barrier; this lane's own rows of
+    //    plane it are taken from registers below.
     {
-      double *czs = coef + slot(it - 1) * SLOT + 2 * TY * 64;
+      double *ks = kr + eslot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        const double k = pk[P][r];
-        cxn[r] = h3d_from_right(k, 0.0) + k;
-        cyn[r] = pk[P][r + 1] + k;
-        czs[r * 64] = kprev[1 - P][r] + k;
-        kprev[P][r] = k;
-        rn[r] = RHS ? pr[P][r] : 0.0;  // rhs of plane it
-      }
-      cye[(eslot(it) * W + w) * 64] = cyn[RY - 1];
+      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[P][r];
     }
-    // 2. Level 0: the loaded plane becomes the newest slot of level 0.
+    double rn[RY];
 #pragma unroll
-    for (int r = 0; r < RY; ++r) A[0][1 - P][r] = pu[P][r];
+    for (int r = 0; r < RY; ++r) rn[r] = RHS ? pr[P][r] : 0.0;  // rhs of plane it
+    // 2. Level 0: the loaded plane becomes the newest slot of level 0.
+    double kq1[RY];  // kappa of plane q+1 (this lane's rows)
+#pragma unroll
+    for (int r = 0; r < RY; ++r) {
+      A[0][1 - P][r] = pu[P][r];
+      kq1[r] = pk[P][r];
+    }
     // 3. Prefetch the next plane (clamped, so always a valid address).
     load_plane(it + 1, 1 - P);
 
@@ -594,8 +594,12 @@ struct Pass64 {
       const bool q_out = (l == KB) && q >= zb && q <= ze;
       const double ub = ue[ue_at(1 - P, l - 1, wb, 1)];  // row below (mid plane)
       const double ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
-      const double cyb = cye[(eslot(q) * W + wb) * 64];
-      const double *cs = coef + slot(q) * SLOT;
+      const int so = eslot(q) * SLOT;
+      double kc[RY];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) kc[r] = kr[so + r * 64];
+      const double k_below = kb[so];
+      const double k_above = ka[so];
       const long long qplane = (long long)(q_out ? q : zb) * sz;
       const auto ro = plane_rsrc(uout + qplane, b8);
       double fy_prev = 0.0;
@@ -603,14 +607,15 @@ struct Pass64 {
       for (int r = 0; r < RY; ++r) {
         const double u = A[l - 1][P][r];
         const double un = A[l - 1][1 - P][r];
-        const double ccx = cs[r * 64];
-        const double ccy = cs[TY * 64 + r * 64];
-        const double ccz = cs[2 * TY * 64 + r * 64];
+        const double k = kc[r];
+        const double ccx = h3d_from_right(k, 0.0) + k;
+        const double ccy = ((r < RY - 1) ? kc[r + 1] : k_above) + k;
+        const double ccz = k + kq1[r];
         const double fxp = ccx * (h3d_from_right(u, 0.0) - u);
         const double fxm = h3d_from_left(fxp, 0.0);
         const double ujp = (r < RY - 1) ? A[l - 1][P][r + 1] : ua;
         const double fyp = ccy * (ujp - u);
-        const double fym = (r > 0) ? fy_prev : cyb * (u - ub);
+        const double fym = (r > 0) ? fy_prev : (k + k_below) * (u - ub);
         fy_prev = fyp;
         const double fzp = ccz * (un - u);
         const double fzm = FZ[l][r];
@@ -628,18 +633,10 @@ struct Pass64 {
           st_f64(ro, s8[r], res);
         }
       }
-    }
-    // 5. Coefficients of plane it into the slot just vacated by it-KB, and
-    //    this iteration's newest edge rows of levels 0..KB-1.
-    {
-      double *cs = coef + slot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        cs[r * 64] = cxn[r];
-        cs[TY * 64 + r * 64] = cyn[r];
-      }
+      for (int r = 0; r < RY; ++r) kq1[r] = kc[r];
     }
-    // rhs ring: plane it joins, plane it-KB leaves
+    // 5. rhs ring: plane it joins, plane it-KB leaves
     if (RHS) {
 #pragma unroll
       for (int l = KB - 1; l > 0; --l)
@@ -648,6 +645,7 @@ struct Pass64 {
 #pragma unroll
       for (int r = 0; r < RY; ++r) RH[0][r] = rn[r];
     }
+    // 6. This iteration's newest edge rows of levels 0..KB-1.
 #pragma unroll
     for (int l = 0; l < KB; ++l) {
       ue[ue_at(P, l, w, 0)] = A[l][1 - P][0];
@@ -689,7 +687,7 @@ __device__ __forceinline__ void pass_f64_tile(
   st.lane_int = i >= 1 && i <= nx - 2;
   const bool lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
 #pragma unroll
-  for (int r = 0; r <= RY; ++r) {
+  for (int r = 0; r < RY; ++r) {
     const int j = j0 + r;
     const int jc = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
     st.o8[r] = (unsigned)(jc * sy + ic) * 8u;
@@ -704,9 +702,14 @@ __device__ __forceinline__ void pass_f64_tile(
     const bool out = lane_out && tr >= KB && tr <= S::TY - 1 - KB && j <= ny - 2;
     st.s8[r] = out ? st.o8[r] : OUT_OF_RANGE;
   }
-  st.coef = lds + st.w * RY * 64 + lane;
-  st.cye = lds + KB * S::SLOT + lane;
-  st.ue = lds + KB * S::SLOT + (KB + 1) * W * 64 + lane;
+  // kappa rows next to this wave's rows: the other waves' edge rows.  The
+  // tile's own edge rows (row 0 of wave 0, the last row of wave W-1) take
+  // their own row instead; their cells are off the output region at every
+  // level above 0, so the value never reaches a stored result.
+  st.kr = lds + st.w * RY * 64 + lane;
+  st.kb = lds + (st.w > 0 ? st.w * RY - 1 : 0) * 64 + lane;
+  st.ka = lds + (st.w < W - 1 ? (st.w + 1) * RY : S::TY - 1) * 64 + lane;
+  st.ue = lds + (KB + 1) * S::SLOT + lane;
 
 #pragma unroll
   for (int l = 0; l <= KB; ++l)
@@ -714,7 +717,6 @@ __device__ __forceinline__ void pass_f64_tile(
     for (int r = 0; r < RY; ++r) st.FZ[l][r] = 0.0;
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
-    st.kprev[0][r] = st.kprev[1][r] = 0.0;
 #pragma unroll
     for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = st.RH[l][r] = 0.0;
   }

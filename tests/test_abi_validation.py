@@ -90,8 +90,6 @@ def test_fp64_pass_checks(h3d, L):
         assert P(u, v, None, k, gp, 0.1, kblock, 1, 14, C.byref(t), None) == EARG
     t = h3d.Tune(2, 8, -1, 2)
     assert P(u, v, None, k, gp, 0.1, 2, 1, 14, C.byref(t), None) == EARG  # z chunk
-    t = h3d.Tune(4, 8, 8, 4)
-    assert P(u, v, r, k, gp, 0.1, 4, 1, 14, C.byref(t), None) == ELIMIT  # LDS above 160 KB
     for ks, ke in ((0, 14), (1, 15), (-3, 2)):  # ring planes or outside the array
         assert P(u, v, None, k, gp, 0.1, 2, ks, ke, None, None) == EARG
     ms, fb = C.c_float(), C.c_int()

@@ -257,7 +257,9 @@ PASS_CASES = [
     ((70, 40, 30), 2, 2, 8, 0, True),
     ((70, 40, 30), 2, 4, 4, 7, True),
     ((65, 33, 21), 3, 2, 8, 5, False),
-    ((65, 33, 21), 3, 2, 8, 5, True),   # exactly 160 KB of LDS
+    ((65, 33, 21), 3, 2, 8, 5, True),
+    ((70, 40, 30), 3, 4, 8, 6, True),   # 32-row tiles at 3 and 4 steps
+    ((70, 70, 17), 4, 4, 8, 0, False),
     ((130, 35, 19), 4, 4, 4, 0, False),
     ((9, 75, 12), 2, 4, 8, 3, False),
     ((9, 75, 12), 3, 4, 4, 3, True),
@@ -391,9 +393,6 @@ def test_fused_pass_argument_errors(gpu):
     t = gpu.Tune(2, 4, 0, 4)  # 8 rows cannot hold a 2*4 halo
     assert L.h3d_pass_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 4, 1, 18, gpu.C.byref(t),
                           None) == -1
-    t = gpu.Tune(4, 8, 0, 4)  # 4 steps of 32 rows with rhs: LDS above 160 KB
-    assert L.h3d_pass_f64(d1.ptr, d2.ptr, d1.ptr, d2.ptr, gpu.C.byref(g), f, 4, 1, 18, gpu.C.byref(t),
-                          None) == -2
     t = gpu.Tune(2, 8, 0, 5)
     ms, fb = gpu.C.c_float(), gpu.C.c_int()
     assert L.h3d_run_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, gpu.C.byref(t), 0,
