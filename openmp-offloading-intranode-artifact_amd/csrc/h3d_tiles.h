@@ -219,25 +219,26 @@ __device__ __forceinline__ void st_u16(__amdgpu_buffer_rsrc_t r, unsigned off,
 }
 
 // LDS of one workgroup, in floats, for KB fused steps and W waves of RY rows:
-//   coef : [KB slots][3: cx, cy, cz][TY rows][64]  plane p in slot p mod KB
-//          (each lane reads only its own cells)
-//   cye  : [KB+1][W][64]   cy of each wave's last row, read by the wave above
-//   ue   : [2][KB][W][2][64]  edge rows of every level, double buffered
+//   kr : [KB+1 slots][TY rows][64]  kappa of planes it-KB .. it (plane p in
+//        slot p mod KB+1)
+//   ue : [2][KB][W][2][64]  edge rows of every level, double buffered
+// Every face sum k_a + k_b is formed where it is used, from the same two
+// operands, so it has the bits a stored sum would have; holding kappa
+// instead of three sums per cell takes KB+1 planes of LDS instead of 3*KB.
 __host__ __device__ constexpr int pass_lds_floats(int KB, int RY, int W) {
-  return KB * 3 * (W * RY) * 64 + (KB + 1) * W * 64 + 2 * KB * W * 2 * 64;
+  return (KB + 1) * (W * RY) * 64 + 2 * KB * W * 2 * 64;
 }
 
 template <int KB, int RY, int W, int MODE, bool BF16>
 struct Pass {
   static constexpr int TY = W * RY;
-  static constexpr int SLOT = 3 * TY * 64;  // floats per coefficient slot
+  static constexpr int SLOT = TY * 64;  // floats per kappa slot
   // ---- per-lane state ----
   float A[KB][2][RY];        // u of level l: slot P = plane it-l-1, 1-P = it-l
   float E[KB][2][RY];        // residual of level l, same slots
   float FZ[KB + 1][RY];      // z flux above the last plane of level l
-  float kprev[2][RY];        // kappa rows of the previous loaded plane
   float pu[2][RY];           // prefetched u rows (slot per phase)
-  float pk[2][RY + 1];       // prefetched kappa rows + the row above
+  float pk[2][RY];           // prefetched kappa rows
   unsigned short pc[2][RY];  // prefetched residual bits (COMP16)
 
   // ---- geometry ----
@@ -250,21 +251,18 @@ struct Pass {
   float factor;
   int nz, w;
   long long sz;
-  unsigned o4[RY + 1];  // byte offsets of rows j0..j0+RY in a float plane
-  unsigned o2[RY + 1];  // the same in a 16-bit plane
-  unsigned s4[RY];      // store offsets: o4, or OUT_OF_RANGE off the output region
-  unsigned s2[RY];      // the same for the 16-bit residual plane
+  unsigned o4[RY];  // byte offsets of rows j0..j0+RY-1 in a float plane
+  unsigned o2[RY];  // the same in a 16-bit plane
+  unsigned s4[RY];  // store offsets: o4, or OUT_OF_RANGE off the output region
+  unsigned s2[RY];  // the same for the 16-bit residual plane
   bool lane_int;
   bool row_int[RY];
   int zb, ze;
-  float *coef;  // this lane, slot 0, cx, this wave's row 0
-  float *cye;   // this lane, slot 0, wave 0
-  float *ue;    // this lane, (phase 0, level 0, wave 0, lo)
+  float *kr;        // this lane, slot 0, this wave's row 0
+  const float *kb;  // this lane, slot 0, the row below this wave's rows
+  const float *ka;  // this lane, slot 0, the row above this wave's rows
+  float *ue;        // this lane, (phase 0, level 0, wave 0, lo)
 
-  __device__ __forceinline__ int slot(int p) const {  // p mod KB
-    const int s = p % KB;
-    return s < 0 ? s + KB : s;
-  }
   __device__ __forceinline__ int eslot(int p) const {  // p mod KB+1
     const int s = p % (KB + 1);
     return s < 0 ? s + KB + 1 : s;
@@ -282,9 +280,6 @@ struct Pass {
     for (int r = 0; r < RY; ++r) {
       pu[ph][r] = ld_f32(ru, o4[r]);
       if (MODE == 2) pc[ph][r] = ld_u16(plane_rsrc(cin + plane, b2), o2[r]);
-    }
-#pragma unroll
-    for (int r = 0; r <= RY; ++r) {
       if (BF16) pk[ph][r] = bf16_to_f32(ld_u16(plane_rsrc(kb16 + plane, b2), o2[r]));
       else pk[ph][r] = ld_f32(plane_rsrc(kf + plane, b4), o4[r]);
     }
@@ -294,27 +289,23 @@ struct Pass {
   template <int P>
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
-    // 1. Face coefficients: cz of plane it-1 now (read by level 1 below),
-    //    cx/cy of plane it at the end (their slot is read below as it-KB).
-    float cxn[RY], cyn[RY];
+    // 1. kappa of plane it into its ring slot (the slot of it-KB-1, which no
+    //    level reads any more).  Other waves read these rows from the next
+    //    iteration on, after the barrier; this lane's own rows of plane it
+    //    are taken from registers below.
     {
-      float *czs = coef + slot(it - 1) * SLOT + 2 * TY * 64;
+      float *ks = kr + eslot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        const float k = pk[P][r];
-        cxn[r] = lane_right(k) + k;
-        cyn[r] = pk[P][r + 1] + k;
-        czs[r * 64] = kprev[1 - P][r] + k;
-        kprev[P][r] = k;
-      }
-      cye[(eslot(it) * W + w) * 64] = cyn[RY - 1];
+      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[P][r];
     }
     // 2. Level 0: the loaded plane becomes the newest slot of level 0.
+    float kq1[RY];  // kappa of plane q+1 (this lane's rows)
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
       A[0][1 - P][r] = pu[P][r];
       if (MODE == 2) E[0][1 - P][r] = comp_decode(pc[P][r], pu[P][r]);
       else E[0][1 - P][r] = 0.0f;
+      kq1[r] = pk[P][r];
     }
     // 3. Prefetch the next plane (clamped, so always a valid address).
     load_plane(it + 1, 1 - P);
@@ -329,8 +320,12 @@ struct Pass {
       const bool q_out = (l == KB) && q >= zb && q <= ze;
       const float ub = ue[ue_at(1 - P, l - 1, wb, 1)];  // row below (mid plane)
       const float ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
-      const float cyb = cye[(eslot(q) * W + wb) * 64];
-      const float *cs = coef + slot(q) * SLOT;
+      const int so = eslot(q) * SLOT;
+      float kc[RY];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) kc[r] = kr[so + r * 64];
+      const float k_below = kb[so];
+      const float k_above = ka[so];
       const long long qplane = (long long)(q_out ? q : zb) * sz;
       const auto ro = plane_rsrc(uout + qplane, (unsigned)sz * 4u);
       const auto rc = plane_rsrc(MODE == 2 ? cout + qplane : nullptr, (unsigned)sz * 2u);
@@ -339,14 +334,15 @@ struct Pass {
       for (int r = 0; r < RY; ++r) {
         const float u = A[l - 1][P][r];
         const float un = A[l - 1][1 - P][r];
-        const float ccx = cs[r * 64];
-        const float ccy = cs[TY * 64 + r * 64];
-        const float ccz = cs[2 * TY * 64 + r * 64];
+        const float k = kc[r];
+        const float ccx = lane_right(k) + k;
+        const float ccy = ((r < RY - 1) ? kc[r + 1] : k_above) + k;
+        const float ccz = k + kq1[r];
         const float fxp = ccx * (lane_right(u) - u);
         const float fxm = lane_left(fxp);
         const float ujp = (r < RY - 1) ? A[l - 1][P][r + 1] : ua;
         const float fyp = ccy * (ujp - u);
-        const float fym = (r > 0) ? fy_prev : cyb * (u - ub);
+        const float fym = (r > 0) ? fy_prev : (k + k_below) * (u - ub);
         fy_prev = fyp;
         const float fzp = ccz * (un - u);
         const float fzm = FZ[l][r];
@@ -379,17 +375,10 @@ struct Pass {
           if (MODE == 2) st_u16(rc, s2[r], comp_encode(e2, res));
         }
       }
-    }
-    // 5. Coefficients of plane it into the slot just vacated by it-KB, and
-    //    this iteration's newest edge rows of levels 0..KB-1.
-    {
-      float *cs = coef + slot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) {
-        cs[r * 64] = cxn[r];
-        cs[TY * 64 + r * 64] = cyn[r];
-      }
+      for (int r = 0; r < RY; ++r) kq1[r] = kc[r];
     }
+    // 5. This iteration's newest edge rows of levels 0..KB-1.
 #pragma unroll
     for (int l = 0; l < KB; ++l) {
       ue[ue_at(P, l, w, 0)] = A[l][1 - P][0];
@@ -433,7 +422,7 @@ __device__ __forceinline__ void pass_f32_tile(
   st.lane_int = i >= 1 && i <= nx - 2;
   const bool lane_out = lane >= KB && lane <= 63 - KB && i <= nx - 2;
 #pragma unroll
-  for (int r = 0; r <= RY; ++r) {
+  for (int r = 0; r < RY; ++r) {
     const int j = j0 + r;
     const int jc = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
     const unsigned e = (unsigned)(jc * sy + ic);
@@ -451,9 +440,14 @@ __device__ __forceinline__ void pass_f32_tile(
     st.s4[r] = out ? st.o4[r] : OUT_OF_RANGE;
     st.s2[r] = out ? st.o2[r] : OUT_OF_RANGE;
   }
-  st.coef = lds + st.w * RY * 64 + lane;
-  st.cye = lds + KB * S::SLOT + lane;
-  st.ue = lds + KB * S::SLOT + (KB + 1) * W * 64 + lane;
+  // kappa rows next to this wave's rows: the other waves' edge rows.  The
+  // tile's own edge rows (row 0 of wave 0, the last row of wave W-1) take
+  // their own row instead; their cells are off the output region at every
+  // level above 0, so the value never reaches a stored result.
+  st.kr = lds + st.w * RY * 64 + lane;
+  st.kb = lds + (st.w > 0 ? st.w * RY - 1 : 0) * 64 + lane;
+  st.ka = lds + (st.w < W - 1 ? (st.w + 1) * RY : S::TY - 1) * 64 + lane;
+  st.ue = lds + (KB + 1) * S::SLOT + lane;
 
 #pragma unroll
   for (int l = 0; l <= KB; ++l)
@@ -461,7 +455,6 @@ __device__ __forceinline__ void pass_f32_tile(
     for (int r = 0; r < RY; ++r) st.FZ[l][r] = 0.0f;
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
-    st.kprev[0][r] = st.kprev[1][r] = 0.0f;
 #pragma unroll
     for (int l = 0; l < KB; ++l)
       st.A[l][0][r] = st.A[l][1][r] = st.E[l][0][r] = st.E[l][1][r] = 0.0f;
