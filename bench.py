@@ -423,15 +423,15 @@ def leg_cfg5(bw, stream):
     for trip in keep + keep2:
         for d in trip:
             d.close()
-    # FP64 members 24 B per update (per 2-step pass at kblock 2); FP32
-    # members 4+4+4 B plus 2+2 B residual per pass of kblock steps
-    k64 = 2 if et.kblock == 2 else 1
+    # FP64 members 24 B and FP32 members 4+4+4 B plus 2+2 B residual per
+    # cell and pass of kblock steps
+    k64 = et.kblock
     b64 = sum(24.0 / k64 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
     b32 = sum(16.0 / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
     gbs = (b64 + b32) * 500 / sec / 1e9
     return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
             "cells": int(cells), "steps_timed": 500, "graph": True, "zchunk": et.zchunk,
-            "fp32_kblock": et.kblock, "fp64_steps_per_pass": k64,
+            "kblock": et.kblock,
             "ms_per_step": round(1e3 * sec / 500, 5),
             "work_items": nwork.value, "workgroups": nblk.value,
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),

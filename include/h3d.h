@@ -287,24 +287,22 @@ typedef struct h3d_ensemble h3d_ensemble;
  * members' residual fields (zeroed).  Tiles are fixed in x and y (256-thread
  * workgroups).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
  * depth (planes per work item, default 64) and tune->kblock the
- * temporal-blocking depth (steps fused per pass of an FP32 member, 1..3,
- * default 1; H3D_ELIMIT above; at 2, FP64 members fuse 2 steps per pass
- * as well). */
+ * temporal-blocking depth (steps fused per pass of every member, 1..3,
+ * default 1; H3D_ELIMIT above). */
 int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
                         int count, const h3d_tune *tune, int device_id);
-/* `steps` time steps of every member.  With kblock 2, each group of 2
- * steps is one launch of 2-step passes for every member; with kblock 3,
- * one launch (FP64 tiles one step, FP32 tiles a 3-step pass) and 2
- * launches over the FP64 tiles; steps past the last full group are
+/* `steps` time steps of every member.  With kblock K > 1, each group of K
+ * steps is one launch of K-step passes for every member (FP64 members
+ * bit-identical to h3d_pass_f64); steps past the last full group are
  * single-step launches.  graph != 0 captures the run's launches into one
- * hipGraph (kept and reused while the step count and starting parities
+ * hipGraph (kept and reused while the step count and starting parity
  * repeat) and launches that.  *ms is the device time of the steps.
- * Members ping-pong u_a -> u_b once per launch that advances them:
+ * Members ping-pong u_a -> u_b once per launch:
  * h3d_ensemble_where says which buffer holds a member's current field. */
 int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph, void *stream,
                      float *ms);
-/* Times the ensemble's step for z-march depths 32..256 planes and, when it
- * has FP32 members, FP32 pass depths 1..3, and keeps the fastest per step
+/* Times the ensemble's step for z-march depths 32..256 planes and pass
+ * depths 1..3, and keeps the fastest per step
  * (written to *out if out != NULL).  The timed launches recompute the next
  * step into the idle buffers, so the members' state and the results of
  * later runs are unchanged. */

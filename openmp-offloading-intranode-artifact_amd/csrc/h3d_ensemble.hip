@@ -14,16 +14,12 @@
 // persistent kernel the LDS tile measured 8% faster than the
 // wave-independent single-step tile, so it is used for kblock = 1 too.
 //
-// With kblock = 2 a group of 2 steps is one launch in which every tile
-// takes a 2-step pass: FP64 tiles run the FP64 pass tile (4 rows per lane,
-// 4 waves), bit-identical to h3d_pass_f64.  With kblock = 3 a group is one
-// launch in which FP64 tiles take one step and FP32 tiles a 3-step pass,
-// then 2 launches over the FP64 tiles only (a 3-step FP64 pass of 4 waves
-// would leave 10 of its 16 rows, and its LDS one workgroup per CU).  Steps
-// beyond the last full group are single-step launches over every tile.
-// Every member flips its ping-pong buffers once per launch that advances
-// it (h3d_ensemble_where).  The launches of a run are captured into one
-// hipGraph on request.
+// With kblock = K > 1 a group of K steps is one launch in which every tile
+// takes a K-step pass: FP64 tiles run the FP64 pass tile (4 rows per lane,
+// 4 waves), bit-identical to h3d_pass_f64.  Steps beyond the last full
+// group are single-step launches over every tile.  Every member flips its
+// ping-pong buffers once per launch (h3d_ensemble_where).  The launches of
+// a run are captured into one hipGraph on request.
 #include <vector>
 #include <algorithm>
 
@@ -35,8 +31,7 @@ constexpr int ENS_WAVES = 4;    // 256-thread workgroups
 constexpr int ENS_F64_ROWS = 2;
 constexpr int ENS_F32_RY = 4;
 constexpr int ENS_ZCHUNK = 64;  // default z-march depth
-constexpr int ENS_MAXKB = 3;    // deepest FP32 pass fused inside the ensemble
-constexpr int ENS_F64_FUSE = 2; // the depth at which FP64 members fuse too
+constexpr int ENS_MAXKB = 3;    // deepest pass fused inside the ensemble
 constexpr int ENS_F64_PRY = 4;  // rows per lane of the FP64 pass tile
 
 struct EMember {
@@ -50,19 +45,19 @@ struct EMember {
   int precision;
 };
 
-// parity64 / parity32: which buffer FP64 / FP32 members read this launch.
+// parity: which buffer every member reads this launch.
 template <int KB>
 __global__ void __launch_bounds__(ENS_WAVES * 64)
 ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
-              int nwork, int parity64, int parity32, int zchunk) {
+              int nwork, int parity, int zchunk) {
   extern __shared__ float lds[];
   for (int t = blockIdx.x; t < nwork; t += gridDim.x) {
     const int4 d = work[t];
     const EMember m = mem[d.x];
     if (m.precision == 0) {
-      const double *in = static_cast<const double *>(parity64 ? m.ub : m.ua);
-      double *out = static_cast<double *>(parity64 ? m.ua : m.ub);
-      if constexpr (KB == ENS_F64_FUSE) {
+      const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
+      double *out = static_cast<double *>(parity ? m.ua : m.ub);
+      if constexpr (KB > 1) {
         const int zb = 1 + d.w * zchunk;
         pass_f64_tile<KB, ENS_F64_PRY, ENS_WAVES, false>(
             reinterpret_cast<double *>(lds), out, in, nullptr,
@@ -75,10 +70,10 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
                                             m.nz - 2, zchunk, 1, d.y, d.z, d.w);
       }
     } else {
-      const float *in = static_cast<const float *>(parity32 ? m.ub : m.ua);
-      float *out = static_cast<float *>(parity32 ? m.ua : m.ub);
-      const unsigned short *cin = parity32 ? m.cb : m.ca;
-      unsigned short *cout = parity32 ? m.ca : m.cb;
+      const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
+      float *out = static_cast<float *>(parity ? m.ua : m.ub);
+      const unsigned short *cin = parity ? m.cb : m.ca;
+      unsigned short *cout = parity ? m.ca : m.cb;
       const int zb = 1 + d.w * zchunk;
       pass_f32_tile<KB, ENS_F32_RY, ENS_WAVES, 2, false>(
           lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
@@ -94,8 +89,7 @@ template <int KB>
 constexpr size_t ens_lds() {
   const size_t f32 = sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
   const size_t f64 =
-      KB == ENS_F64_FUSE ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY, ENS_WAVES)
-                         : 0;
+      KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY, ENS_WAVES) : 0;
   return f32 > f64 ? f32 : f64;
 }
 
@@ -107,25 +101,24 @@ struct h3d_ensemble {
   int zchunk;
   int kb;           // FP32 steps per pass
   long long done;   // steps taken so far
-  long long passes; // FP32 passes taken so far
-  long long flips64; // FP64 buffer flips so far (steps, or passes at depth 2)
+  long long passes; // passes taken so far (every member flips once per pass)
   EMember *d_mem;
   struct Tiles {
+    // tiles per axis at pass depth d (FP64 members: [1] is the step tile)
     int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision;
-    int f64p_x, f64p_y;  // FP64 pass tiling at depth ENS_F64_FUSE
   };
-  std::vector<Tiles> tiles;  // per member; [0] FP64 steps, [d] FP32 at depth d
+  std::vector<Tiles> tiles;  // per member
   // Work lists in one device buffer: every tile at depth kb, every tile at
-  // depth 1 (single-step launches), FP64 tiles only.
+  // depth 1 (single-step launches).
   int4 *d_work;
   int work_cap;
-  int off_kb, n_kb, off_1, n_1, off_64, n_64;
+  int off_kb, n_kb, off_1, n_1;
   int blocks_kb, blocks_1;  // persistent grid of each kernel instance
   std::vector<unsigned short *> comp;  // owned FP16 residual fields
-  // cached graph of `graph_steps` steps starting at parities (p64, p32)
+  // cached graph of `graph_steps` steps starting at buffer parity graph_p
   hipGraphExec_t graph = nullptr;
   hipGraph_t graph_def = nullptr;
-  int graph_steps = -1, graph_p64 = -1, graph_p32 = -1;
+  int graph_steps = -1, graph_p = -1;
 };
 
 static void drop_graph(h3d_ensemble *e) {
@@ -133,7 +126,7 @@ static void drop_graph(h3d_ensemble *e) {
   if (e->graph_def) (void)hipGraphDestroy(e->graph_def);
   e->graph = nullptr;
   e->graph_def = nullptr;
-  e->graph_steps = e->graph_p64 = e->graph_p32 = -1;
+  e->graph_steps = e->graph_p = -1;
 }
 
 static void ensemble_free(h3d_ensemble *e) {
@@ -169,19 +162,18 @@ static int occupancy_blocks(int device, int kb, int nwork, int *blocks) {
 static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   if (zchunk < 1 || kb < 1 || kb > ENS_MAXKB) return H3D_EARG;
   struct Job { long long cost; int4 d; };
-  auto list = [&](int depth, bool f32) {
+  auto list = [&](int depth) {
     std::vector<Job> jobs;
     for (int m = 0; m < e->count; ++m) {
       const h3d_ensemble::Tiles &t = e->tiles[m];
-      if (t.precision == 1 && !f32) continue;
-      const bool fused64 = !t.precision && depth == ENS_F64_FUSE;
-      const int tx_n = t.precision ? t.tiles_x[depth] : fused64 ? t.f64p_x : t.tiles_x[0];
-      const int ty_n = t.precision ? t.tiles_y[depth] : fused64 ? t.f64p_y : t.tiles_y[0];
+      const int tx_n = t.tiles_x[depth], ty_n = t.tiles_y[depth];
       const int chunks = (t.planes + zchunk - 1) / zchunk;
       for (int tz = 0; tz < chunks; ++tz) {
         const int planes = std::min(zchunk, t.planes - tz * zchunk);
-        // relative cost: HBM bytes per cell, 8 per plane and field
-        const long long cost = (long long)planes * (t.precision ? 2 * depth : fused64 ? 4 : 3);
+        // relative cost per plane: FP64 steps 3, FP64 passes 2 + depth,
+        // FP32 passes 2 * depth
+        const long long cost =
+            (long long)planes * (t.precision ? 2 * depth : depth > 1 ? 2 + depth : 3);
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
       }
@@ -192,18 +184,14 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
     for (size_t j = 0; j < jobs.size(); ++j) w[j] = jobs[j].d;
     return w;
   };
-  std::vector<int4> wkb = list(kb, true);
-  std::vector<int4> w1 = kb > 1 ? list(1, true) : std::vector<int4>();
-  // FP64-only single steps between fused FP32 passes (FP64 tiles fuse at
-  // depth ENS_F64_FUSE themselves)
-  std::vector<int4> w64 = kb > 1 && kb != ENS_F64_FUSE ? list(1, false) : std::vector<int4>();
-  const size_t total = wkb.size() + w1.size() + w64.size();
+  std::vector<int4> wkb = list(kb);
+  std::vector<int4> w1 = kb > 1 ? list(1) : std::vector<int4>();
+  const size_t total = wkb.size() + w1.size();
   if (total > 0x7fffffffu) return H3D_ELIMIT;
   std::vector<int4> all;
   all.reserve(total);
   all.insert(all.end(), wkb.begin(), wkb.end());
   all.insert(all.end(), w1.begin(), w1.end());
-  all.insert(all.end(), w64.begin(), w64.end());
   if ((int)total > e->work_cap) {
     if (e->d_work) (void)hipFree(e->d_work);
     e->d_work = nullptr;
@@ -218,8 +206,6 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   // with kb = 1 the single-step list is the kb list itself
   e->off_1 = kb > 1 ? e->n_kb : 0;
   e->n_1 = kb > 1 ? (int)w1.size() : e->n_kb;
-  e->off_64 = e->n_kb + (int)w1.size();
-  e->n_64 = (int)w64.size();
   int rc = occupancy_blocks(e->device, kb, e->n_kb, &e->blocks_kb);
   if (rc == H3D_OK) rc = occupancy_blocks(e->device, 1, e->n_1, &e->blocks_1);
   if (rc != H3D_OK) return rc;
@@ -229,11 +215,11 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   return H3D_OK;
 }
 
-// One launch over list 0 (every tile, FP32 passes of kb steps), 1 (every
-// tile, single steps) or 2 (FP64 tiles only).
-static int ens_launch(h3d_ensemble *e, int list, int p64, int p32, hipStream_t s) {
-  const int4 *w = e->d_work + (list == 0 ? e->off_kb : list == 1 ? e->off_1 : e->off_64);
-  const int n = list == 0 ? e->n_kb : list == 1 ? e->n_1 : e->n_64;
+// One launch over list 0 (every tile, passes of kb steps) or 1 (every
+// tile, single steps).
+static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
+  const int4 *w = e->d_work + (list == 0 ? e->off_kb : e->off_1);
+  const int n = list == 0 ? e->n_kb : e->n_1;
   if (n == 0) return H3D_OK;
   const int depth = list == 0 ? e->kb : 1;
   const int blocks = list == 0 ? e->blocks_kb : e->blocks_1;
@@ -242,50 +228,42 @@ static int ens_launch(h3d_ensemble *e, int list, int p64, int p32, hipStream_t s
     const int big_rc = h3d_allow_big_lds((const void *)ensemble_step<2>, &big_lds_done);
     if (big_rc != H3D_OK) return big_rc;
   }
+  if (depth == 3 && ens_lds<3>() > 64 * 1024) {
+    static std::atomic<unsigned long long> big_lds_done3{0};
+    const int big_rc = h3d_allow_big_lds((const void *)ensemble_step<3>, &big_lds_done3);
+    if (big_rc != H3D_OK) return big_rc;
+  }
   switch (depth) {
     case 1:
       hipLaunchKernelGGL(ensemble_step<1>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<1>(),
-                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+                         s, e->d_mem, w, n, parity, e->zchunk);
       break;
     case 2:
       hipLaunchKernelGGL(ensemble_step<2>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<2>(),
-                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+                         s, e->d_mem, w, n, parity, e->zchunk);
       break;
     default:
       hipLaunchKernelGGL(ensemble_step<3>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<3>(),
-                         s, e->d_mem, w, n, p64, p32, e->zchunk);
+                         s, e->d_mem, w, n, parity, e->zchunk);
       break;
   }
   return h3d_launch_status();
 }
 
-// FP32 passes in a run of `steps` steps: full groups plus single steps.
+// Passes (launches) in a run of `steps` steps: full groups plus single
+// steps.
 static int ens_passes(const h3d_ensemble *e, int steps) {
   const int full = e->kb > 1 ? steps / e->kb : 0;
   return full + (steps - full * e->kb);
 }
 
-// FP64 buffer flips in a run of `steps` steps: one per step, except that a
-// full group at depth ENS_F64_FUSE is one pass.
-static int ens_flips64(const h3d_ensemble *e, int steps) {
-  return e->kb == ENS_F64_FUSE ? ens_passes(e, steps) : steps;
-}
-
-// Issues the launches of `steps` steps from parities (p64, p32).
-static int ens_issue(h3d_ensemble *e, int steps, int p64, int p32, hipStream_t s) {
-  int rc = H3D_OK, np = 0, n64 = 0, k = 0;
-  if (e->kb > 1) {
-    for (; rc == H3D_OK && k + e->kb <= steps; k += e->kb) {
-      rc = ens_launch(e, 0, (p64 + n64) % 2, (p32 + np) % 2, s);
-      ++np;
-      ++n64;
-      if (e->kb == ENS_F64_FUSE) continue;  // FP64 tiles took the whole group
-      for (int j = 1; rc == H3D_OK && j < e->kb; ++j, ++n64)
-        rc = ens_launch(e, 2, (p64 + n64) % 2, (p32 + np) % 2, s);
-    }
-  }
-  for (; rc == H3D_OK && k < steps; ++k, ++np, ++n64)
-    rc = ens_launch(e, 1, (p64 + n64) % 2, (p32 + np) % 2, s);
+// Issues the launches of `steps` steps from buffer parity p.
+static int ens_issue(h3d_ensemble *e, int steps, int p, hipStream_t s) {
+  int rc = H3D_OK, np = 0, k = 0;
+  if (e->kb > 1)
+    for (; rc == H3D_OK && k + e->kb <= steps; k += e->kb, ++np)
+      rc = ens_launch(e, 0, (p + np) % 2, s);
+  for (; rc == H3D_OK && k < steps; ++k, ++np) rc = ens_launch(e, 1, (p + np) % 2, s);
   return rc;
 }
 
@@ -324,17 +302,16 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
     h3d_ensemble::Tiles t{};
     t.planes = x.g.nz - 2;
     t.precision = x.precision;
-    t.tiles_x[0] = (x.g.nx - 2 + 63) / 64;
-    t.tiles_y[0] = (x.g.ny - 2 + ENS_F64_ROWS * ENS_WAVES - 1) / (ENS_F64_ROWS * ENS_WAVES);
     for (int d = 1; d <= ENS_MAXKB; ++d) {
-      const int OX = 64 - 2 * d, OY = ENS_F32_RY * ENS_WAVES - 2 * d;
+      // FP64 single steps: the step tile, 64 x (rows x waves) cells; every
+      // pass tile (FP32 at any depth, FP64 from depth 2) keeps
+      // (64 - 2d) x (RY x waves - 2d) of its cells
+      const bool step64 = x.precision == 0 && d == 1;
+      const int ry = x.precision ? ENS_F32_RY : ENS_F64_PRY;
+      const int OX = step64 ? 64 : 64 - 2 * d;
+      const int OY = step64 ? ENS_F64_ROWS * ENS_WAVES : ry * ENS_WAVES - 2 * d;
       t.tiles_x[d] = (x.g.nx - 2 + OX - 1) / OX;
       t.tiles_y[d] = (x.g.ny - 2 + OY - 1) / OY;
-    }
-    {
-      const int OX = 64 - 2 * ENS_F64_FUSE, OY = ENS_F64_PRY * ENS_WAVES - 2 * ENS_F64_FUSE;
-      t.f64p_x = (x.g.nx - 2 + OX - 1) / OX;
-      t.f64p_y = (x.g.ny - 2 + OY - 1) / OY;
     }
     if (x.precision == 1) {
       // residual fields start at zero; they live as long as the ensemble
@@ -369,12 +346,11 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
   H3D_TRY(hipSetDevice(e->device));
-  const int p64 = (int)(e->flips64 % 2), p32 = (int)(e->passes % 2);
-  if (graph && steps > 0 &&
-      (e->graph_steps != steps || e->graph_p64 != p64 || e->graph_p32 != p32 || !e->graph)) {
+  const int p = (int)(e->passes % 2);
+  if (graph && steps > 0 && (e->graph_steps != steps || e->graph_p != p || !e->graph)) {
     drop_graph(e);
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int rc = ens_issue(e, steps, p64, p32, s);
+    int rc = ens_issue(e, steps, p, s);
     hipGraph_t gdef = nullptr;
     hipError_t ec = hipStreamEndCapture(s, &gdef);
     if (rc != H3D_OK) { if (gdef) (void)hipGraphDestroy(gdef); return rc; }
@@ -382,8 +358,7 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
     e->graph_def = gdef;
     H3D_TRY(hipGraphInstantiate(&e->graph, gdef, nullptr, nullptr, 0));
     e->graph_steps = steps;
-    e->graph_p64 = p64;
-    e->graph_p32 = p32;
+    e->graph_p = p;
   }
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
@@ -393,7 +368,7 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
     if (graph) {
       H3D_TRY(hipGraphLaunch(e->graph, s));
     } else {
-      const int rc = ens_issue(e, steps, p64, p32, s);
+      const int rc = ens_issue(e, steps, p, s);
       if (rc != H3D_OK) return rc;
     }
   }
@@ -404,31 +379,29 @@ extern "C" int h3d_ensemble_run(h3d_ensemble *e, int steps, int graph,
   if (ms) *ms = t;
   e->done += steps;
   e->passes += ens_passes(e, steps);
-  e->flips64 += ens_flips64(e, steps);
   return H3D_OK;
 }
 
 // Times candidate z-march depths and pass depths 1..3, and keeps the
-// fastest per step.  A depth-K group
-// costs one launch over every tile plus K-1 launches over the FP64 tiles;
-// each is timed on its own with the parities of the ensemble's next step,
-// so it reads the current fields and rewrites only the idle buffers with
-// values the next run recomputes: member state is unchanged.
+// fastest per step.  A depth-K group is one launch over every tile, timed
+// with the buffer parity of the ensemble's next step, so it reads the
+// current fields and rewrites only the idle buffers with values the next
+// run recomputes: member state is unchanged.
 extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *out) {
   if (!e) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   H3D_TRY(hipSetDevice(e->device));
-  const int p64 = (int)(e->flips64 % 2), p32 = (int)(e->passes % 2);
+  const int p = (int)(e->passes % 2);
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
   auto time_list = [&](int list, float *best) {
-    int rc = ens_launch(e, list, p64, p32, s);  // warm-up
+    int rc = ens_launch(e, list, p, s);  // warm-up
     *best = 1e30f;
     for (int r = 0; r < 3 && rc == H3D_OK; ++r) {
       float ms = 0.f;
       hipError_t he = hipEventRecord(R.e0, s);
-      if (he == hipSuccess) rc = ens_launch(e, list, p64, p32, s);
+      if (he == hipSuccess) rc = ens_launch(e, list, p, s);
       if (rc != H3D_OK) break;
       if (he == hipSuccess) he = hipEventRecord(R.e1, s);
       if (he == hipSuccess) he = hipEventSynchronize(R.e1);
@@ -445,11 +418,10 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
     for (int zc : {32, 64, 128, 256}) {
       rc = build_work(e, zc, kb);
       if (rc != H3D_OK) break;
-      float tg = 0.f, t64 = 0.f;
+      float tg = 0.f;
       rc = time_list(0, &tg);
-      if (rc == H3D_OK && kb > 1) rc = time_list(2, &t64);
       if (rc != H3D_OK) break;
-      const float per_step = (tg + (kb - 1) * t64) / kb;
+      const float per_step = tg / kb;
       if (per_step < best) { best = per_step; best_zc = zc; best_kb = kb; }
     }
   if (rc != H3D_OK) return rc;
@@ -466,8 +438,7 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
 
 extern "C" int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b) {
   if (!e || !in_b || member < 0 || member >= e->count) return H3D_EARG;
-  const long long flips = e->tiles[member].precision ? e->passes : e->flips64;
-  *in_b = (int)(flips % 2);
+  *in_b = (int)(e->passes % 2);
   return H3D_OK;
 }
 

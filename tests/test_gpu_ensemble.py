@@ -131,11 +131,10 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
 @pytest.mark.parametrize("graph", [0, 1])
 @pytest.mark.parametrize("kb", [2, 3])
 def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
-    """Members fused kb steps per pass (FP64 members fused too at kb = 2,
-    stepping alone between FP32 passes at kb = 3): each FP32 member equals
-    the oracle replaying the same passes with the FP16 residual carried
-    across them, and each FP64 member its plain run, over runs whose
-    lengths are not multiples of kb."""
+    """Every member fused kb steps per pass: each FP32 member equals the
+    oracle replaying the same passes with the FP16 residual carried across
+    them, and each FP64 member its plain run, over runs whose lengths are
+    not multiples of kb."""
     specs = [make_member(gpu, d, m, 13, p) for m, (d, p) in enumerate(SPECS)]
     mems, keep = [], []
     for sp in specs:
@@ -156,8 +155,8 @@ def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
         got = (db if where(gpu, e, m) else da).download()
         g = sp["g"]
         if sp["precision"] == 0:
-            # FP64 tiles fuse at depth 2 (one flip per pass), else step alone
-            assert where(gpu, e, m) == (len(depths) if kb == 2 else steps) % 2
+            # one flip per pass, as for FP32 members
+            assert where(gpu, e, m) == len(depths) % 2
             want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
                                   g.nz, steps, factor=sp["factor"])
             assert np.array_equal(got, want), g.nx
