@@ -48,7 +48,8 @@ typedef struct h3d_grid {
  *   FP64 step and FP32 single-step pass: rows 1, 2, 4 (FP64 also 8),
  *     waves 1..16 for FP64, 1..8 for FP32;
  *   FP32 fused passes (kblock 2..6) and FP64 fused passes (kblock 2..4):
- *     rows 2 or 4, waves 4 or 8. */
+ *     rows 2 or 4, waves 4 or 8; FP64 passes of 2 or 3 steps also rows 2
+ *     with waves 16. */
 typedef struct h3d_tune {
   int rows;   /* y rows per lane held in registers                   */
   int waves;  /* wavefronts per workgroup, stacked in y               */
@@ -103,9 +104,10 @@ int h3d_step_f64(double *u_new, const double *u_old, const double *rhs,
  * Planes are read once and written once per pass, 24/nsteps B per update
  * (32/nsteps with rhs).  u_out's Dirichlet ring must hold u_in's.  No
  * reference counterpart: the reference runs one kernel launch per step
- * (1-openmp.c:193-205).  Shapes: rows 2 or 4, waves 4 or 8, any zchunk
- * (H3D_ELIMIT when the shape's LDS exceeds 160 KB).  nsteps == 1 is
- * h3d_step_f64 over the same planes. */
+ * (1-openmp.c:193-205).  Shapes: rows 2 or 4, waves 4 or 8, and rows 2
+ * with waves 16 at nsteps 2 and 3; any zchunk (H3D_ELIMIT when the
+ * shape's LDS exceeds 160 KB).  nsteps == 1 is h3d_step_f64 over the same
+ * planes. */
 int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
                  const double *kappa, const h3d_grid *g, double factor,
                  int nsteps, int k_start, int k_end, const h3d_tune *tune,

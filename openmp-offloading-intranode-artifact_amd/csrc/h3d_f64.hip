@@ -141,6 +141,11 @@ int launch_pass_kb(int ry, int waves, double *uo, const double *ui, const double
                : launch_pass_one<KB, RY, W, false>(uo, ui, rhs, kf, g, factor, ks, ke, \
                                                    zchunk, s);
   H3D_P(2, 4) H3D_P(2, 8) H3D_P(4, 4) H3D_P(4, 8)
+  // 32 rows as 16 waves of 2: four waves per SIMD where rows 4 x 8 waves
+  // run two (160 KB of LDS at 3 steps; 208 KB at 4 would not fit)
+  if constexpr (KB <= 3) {
+    H3D_P(2, 16)
+  }
 #undef H3D_P
   return H3D_EARG;
 }

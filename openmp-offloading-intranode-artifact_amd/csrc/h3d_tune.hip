@@ -125,7 +125,8 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       // re-reads cost more than the extra parallelism of short chunks
       // buys); fused passes: the LDS-shared tiles with short z chunks
       const std::vector<int> rws = kbk == 1 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4};
-      const std::vector<int> wvs = kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8};
+      const std::vector<int> wvs =
+          kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8, 16};
       const std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
                                             : std::vector<int>{32, 64, 128};
       for (int rows : rws)

@@ -85,7 +85,8 @@ def test_fp64_pass_checks(h3d, L):
     assert P(u, v, None, None, gp, 0.1, 2, 1, 14, None, None) == EARG
     assert P(u, v, None, k, gp, 0.1, 0, 1, 14, None, None) == ELIMIT
     assert P(u, v, None, k, gp, 0.1, 5, 1, 14, None, None) == ELIMIT
-    for kblock, rows, waves in ((2, 1, 4), (2, 2, 2), (3, 8, 8), (4, 2, 4), (2, 2, 16)):
+    for kblock, rows, waves in ((2, 1, 4), (2, 2, 2), (3, 8, 8), (4, 2, 4), (4, 2, 16),
+                                (2, 4, 16)):
         t = h3d.Tune(rows, waves, 8, kblock)
         assert P(u, v, None, k, gp, 0.1, kblock, 1, 14, C.byref(t), None) == EARG
     t = h3d.Tune(2, 8, -1, 2)

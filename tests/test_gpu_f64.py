@@ -260,6 +260,8 @@ PASS_CASES = [
     ((65, 33, 21), 3, 2, 8, 5, True),
     ((70, 40, 30), 3, 4, 8, 6, True),   # 32-row tiles at 3 and 4 steps
     ((70, 70, 17), 4, 4, 8, 0, False),
+    ((70, 40, 30), 3, 2, 16, 7, True),  # 32 rows as 16 waves of 2 (160 KB)
+    ((66, 66, 40), 2, 2, 16, 0, False),
     ((130, 35, 19), 4, 4, 4, 0, False),
     ((9, 75, 12), 2, 4, 8, 3, False),
     ((9, 75, 12), 3, 4, 4, 3, True),
@@ -304,6 +306,7 @@ RANGE_CASES = [
     ((33, 40, 20), 4, 4, 15, 4, 4, 3, False, 0),
     ((40, 30, 24), 1, 6, 9, 2, 2, 0, True, 0),     # one step: h3d_step_f64
     ((33, 40, 20), 4, 1, 18, 0, 0, 0, True, 0),    # default shape at 4 steps
+    ((40, 30, 24), 3, 5, 11, 2, 16, 5, True, 0),   # 16 waves of 2 rows
 ]
 
 
