@@ -30,14 +30,15 @@
 //  - y neighbours are the lane's other rows, except across wavefronts, where
 //    each wave publishes the two edge rows of every level in LDS (double
 //    buffered, one barrier per march iteration);
-//  - face coefficients cx = k(i+1)+k(i), cy, cz are computed once per plane
-//    when kappa is loaded and kept in an LDS ring of KB planes (own cells
-//    only; the cy of each wave's last row, which the wave above needs, has
-//    its own ring of KB+1 planes);
+//  - kappa is kept in an LDS ring of KB+1 planes, and the face sums
+//    cx = k(i+1)+k(i), cy, cz are formed where they are used (the bits a
+//    stored sum would have; a ring of the three sums would take 3*KB
+//    planes); the rows next to a wave's rows come from the neighbouring
+//    waves' slots, written an iteration earlier;
 //  - loads are unconditional, with row/column/plane indices clamped into
 //    the array: a cell outside it only ever feeds cells that are outside
 //    the tile's valid region, and the per-lane in-plane offsets are 32-bit
-//    constants, so a plane costs RY u loads and RY+1 kappa loads and no
+//    constants, so a plane costs RY u loads and RY kappa loads and no
 //    address arithmetic.
 // Residual e of each cell is carried through the KB steps in registers; in
 // COMP16 mode it is also read at the start of the pass and written at the

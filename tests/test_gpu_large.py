@@ -32,15 +32,20 @@ def run64(gpu, P, u0, k, steps, tune=None):
     return (db, da) if fb.value else (da, db)
 
 
-@pytest.mark.parametrize("kb", [2, 3])
-def test_1000_steps_fused_passes_equal_single_steps(gpu, big, kb):
+@pytest.fixture(scope="module")
+def single_1000(gpu, big):
+    a, _ = run64(gpu, big, big["u"], big["k"], 1000)
+    return a.download()
+
+
+@pytest.mark.parametrize("kb,rows,waves", [(2, 4, 8), (3, 2, 8), (3, 4, 8), (3, 2, 16)],
+                         ids=str)
+def test_1000_steps_fused_passes_equal_single_steps(gpu, big, single_1000, kb, rows, waves):
     """The headline configuration as the benchmark runs it (fused FP64
     passes, the tuner's 512^3 shapes) ends 1000 steps on the same bits as
     1000 single steps."""
-    rows = 4 if kb == 2 else 2
-    a, _ = run64(gpu, big, big["u"], big["k"], 1000)
-    b, _ = run64(gpu, big, big["u"], big["k"], 1000, gpu.Tune(rows, 8, 128, kb))
-    assert a.download().tobytes() == b.download().tobytes()
+    b, _ = run64(gpu, big, big["u"], big["k"], 1000, gpu.Tune(rows, waves, 128, kb))
+    assert single_1000.tobytes() == b.download().tobytes()
 
 
 def test_full_field_bitwise_vs_oracle(gpu, oracle, big):
