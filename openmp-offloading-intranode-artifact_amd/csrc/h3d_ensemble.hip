@@ -139,9 +139,24 @@ static void ensemble_free(h3d_ensemble *e) {
   delete e;
 }
 
+// Kernels whose LDS exceeds the 64 KB default need the attribute raised on
+// each device before they are sized or launched.
+template <int KB>
+static int allow_lds() {
+  if (ens_lds<KB>() <= 64 * 1024) return H3D_OK;
+  static std::atomic<unsigned long long> done{0};
+  return h3d_allow_big_lds((const void *)ensemble_step<KB>, &done);
+}
+
+static int allow_lds_kb(int kb) {
+  return kb == 1 ? allow_lds<1>() : kb == 2 ? allow_lds<2>() : allow_lds<3>();
+}
+
 static int occupancy_blocks(int device, int kb, int nwork, int *blocks) {
   int cus = 0, per_cu = 0;
   H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  const int lrc = allow_lds_kb(kb);
+  if (lrc != H3D_OK) return lrc;
   hipError_t he;
   switch (kb) {
     case 1: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -223,16 +238,8 @@ static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
   if (n == 0) return H3D_OK;
   const int depth = list == 0 ? e->kb : 1;
   const int blocks = list == 0 ? e->blocks_kb : e->blocks_1;
-  if (depth == 2 && ens_lds<2>() > 64 * 1024) {
-    static std::atomic<unsigned long long> big_lds_done{0};
-    const int big_rc = h3d_allow_big_lds((const void *)ensemble_step<2>, &big_lds_done);
-    if (big_rc != H3D_OK) return big_rc;
-  }
-  if (depth == 3 && ens_lds<3>() > 64 * 1024) {
-    static std::atomic<unsigned long long> big_lds_done3{0};
-    const int big_rc = h3d_allow_big_lds((const void *)ensemble_step<3>, &big_lds_done3);
-    if (big_rc != H3D_OK) return big_rc;
-  }
+  const int lrc = allow_lds_kb(depth);
+  if (lrc != H3D_OK) return lrc;
   switch (depth) {
     case 1:
       hipLaunchKernelGGL(ensemble_step<1>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<1>(),
