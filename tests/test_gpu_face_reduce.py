@@ -163,17 +163,18 @@ def test_band_pack_unpack_bitwise(gpu, dims, pitch, k, count):
     assert du.download().tobytes() == want.ravel().tobytes()
 
 
-def deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, steps):
+def deep_standin_oracle(oracle, u, rhs, k, g, factor, ghost, steps):
     """Passes of `ghost` steps (the last one shorter) of the whole slab
     array, each followed by the periodic band copies: owned planes
     nz-2*ghost..nz-1-ghost into ghosts 0..ghost-1, owned planes
     ghost..2*ghost-1 into ghosts nz-ghost..nz-1."""
-    P = n * n
+    P, nz = g.sz, g.nz
     a = u.copy()
     left = steps
     while left:
         r = min(ghost, left)
-        a = oracle.run_f64(a.copy(), a.copy(), rhs, k, n, n, nz, r, factor=factor).copy()
+        a = oracle.run_f64(a.copy(), a.copy(), rhs, k, g.nx, g.ny, nz, r, sy=g.sy, sz=g.sz,
+                           factor=factor).copy()
         a[0:ghost * P] = a[(nz - 2 * ghost) * P:(nz - ghost) * P]
         a[(nz - ghost) * P:] = a[ghost * P:2 * ghost * P]
         left -= r
@@ -181,20 +182,22 @@ def deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, steps):
 
 
 @pytest.mark.parametrize("graph", [0, 1])
-@pytest.mark.parametrize("ghost,nz", [(1, 5), (2, 6), (2, 9), (2, 23), (3, 9), (3, 16),
-                                      (4, 30)], ids=str)
-def test_deep_slab_standin_matches_oracle(gpu, oracle, graph, ghost, nz):
+@pytest.mark.parametrize("ghost,nz,pitch", [(1, 5, 0), (2, 6, 0), (2, 9, 0), (2, 23, 0),
+                                            (3, 9, 0), (3, 16, 0), (4, 30, 0), (2, 12, 11),
+                                            (3, 16, 27)], ids=str)
+def test_deep_slab_standin_matches_oracle(gpu, oracle, graph, ghost, nz, pitch):
     """h3d_run_slab_pass_f64 (fused passes of `ghost` steps, boundary bands
     first, interior on a second stream, band exchange into the slab's own
     ghosts) against the oracle stepping the whole slab array and copying
-    the bands; with ghost 1 it is h3d_run_slab_f64."""
+    the bands, on dense and padded rows; with ghost 1 it is
+    h3d_run_slab_f64."""
     n, steps = 37, 2 * ghost + 3
-    g = gpu.Grid.dense(n, n, nz)
+    g = gpu.Grid(n, n, nz, n + pitch, (n + pitch) * n)
     u, k = gpu.gen_seeded(g, 0, n, n, nz, 1, 5)
     u[:] = np.where(u == 0.0, 0.5, u)  # ghosts hold data, as in a real slab
     rhs = np.random.default_rng(6).uniform(0, 1e-4, g.elems)
     factor = 0.02
-    want = deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, steps)
+    want = deep_standin_oracle(oracle, u, rhs, k, g, factor, ghost, steps)
     da, db, dk, dr = (gpu.DeviceArray.from_host(x) for x in (u, u, k, rhs))
     lo, hi = gpu.DeviceArray(ghost * n * n * 8), gpu.DeviceArray(ghost * n * n * 8)
     s = gpu.Stream()
@@ -218,7 +221,7 @@ def test_deep_slab_standin_matches_oracle(gpu, oracle, graph, ghost, nz):
              ghost, r, lo.ptr, hi.ptr, None, s.ptr)
     s.sync()
     assert np.array_equal(one.download(),
-                          deep_standin_oracle(oracle, u, rhs, k, n, nz, factor, ghost, r))
+                          deep_standin_oracle(oracle, u, rhs, k, g, factor, ghost, r))
 
 
 @pytest.mark.parametrize("parts,ghost", [(2, 2), (4, 2), (4, 3), (2, 4), (4, 4)], ids=str)
