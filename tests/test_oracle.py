@@ -209,6 +209,60 @@ def test_slab_partition_init_matches_global(oracle):
         assert np.array_equal(su, u[lo:lo + nz_l * n * n])
 
 
+def deep_slab_run(oracle, n, parts, ghost, steps, shift=0):
+    """The deep-ghost exchange on the CPU: the reference's split widened to
+    `ghost` ghost planes per side (ending at the global Dirichlet planes),
+    passes of up to `ghost` oracle steps over each slab array, then the
+    top owned band of slab p into the lower ghosts of p+1 and the bottom
+    owned band of p+1 into the upper ghosts of p; `shift` moves the first
+    band down by that many planes (a planted mistake).  Returns the owned
+    planes reassembled into the global array."""
+    h, dt, factor = oracle.params_reference(n)
+    u, _, rhs, k = oracle.init_reference(n)
+    rc, nzd, gks = oracle.slab_decompose(n, parts)
+    assert rc == 0
+    P = n * n
+    slabs = []
+    for nz_l, gk in zip(nzd, gks):
+        own_lo, own_hi = gk, gk + nz_l - 3
+        start, end = max(0, own_lo - ghost), min(n - 1, own_hi + ghost)
+        sl = slice(start * P, (end + 1) * P)
+        slabs.append(dict(nz=end - start + 1, start=start, lo=own_lo - start, hi=own_hi - start,
+                          old=u[sl].copy(), new=u[sl].copy(), rhs=rhs[sl], kap=k[sl]))
+    left = steps
+    while left:
+        r = min(ghost, left)
+        for s in slabs:
+            full = oracle.run_f64(s["old"].copy(), s["old"].copy(), s["rhs"], s["kap"], n, n,
+                                  s["nz"], r, factor=factor)
+            s["new"][s["lo"] * P:(s["hi"] + 1) * P] = full[s["lo"] * P:(s["hi"] + 1) * P]
+        for a, b in zip(slabs, slabs[1:]):
+            top = a["hi"] - ghost + 1 - shift
+            b["new"][:ghost * P] = a["new"][top * P:(top + ghost) * P]
+            a["new"][(a["hi"] + 1) * P:(a["hi"] + 1 + ghost) * P] = \
+                b["new"][b["lo"] * P:(b["lo"] + ghost) * P]
+        for s in slabs:
+            s["old"], s["new"] = s["new"], s["old"]
+        left -= r
+    out = u.copy()
+    for s in slabs:
+        g0 = s["start"]
+        out[(g0 + s["lo"]) * P:(g0 + s["hi"] + 1) * P] = s["old"][s["lo"] * P:(s["hi"] + 1) * P]
+    return out, oracle.run_f64(u.copy(), u.copy(), rhs, k, n, n, n, steps, factor=factor)
+
+
+@pytest.mark.parametrize("parts,ghost", [(2, 2), (4, 2), (4, 3), (2, 4)], ids=str)
+def test_deep_ghost_exchange_equals_global_run(oracle, parts, ghost):
+    """Slabs with `ghost` ghost planes, advanced `ghost` steps per pass with
+    one band exchange per pass, reassemble to the global run bit for bit;
+    a band taken one plane too low breaks it (the check the GPU test of the
+    same procedure relies on)."""
+    got, want = deep_slab_run(oracle, 38, parts, ghost, 2 * ghost + 1)
+    assert np.array_equal(got, want)
+    bad, _ = deep_slab_run(oracle, 38, parts, ghost, 2 * ghost + 1, shift=1)
+    assert not np.array_equal(bad, want)
+
+
 def test_metrics_hand_vectors(oracle):
     ref = np.array([1.0, -2.0, 0.0, 4.0, 1e-13])
     cand = np.array([1.0, -2.5, 1e-7, 4.0, 2e-13])
