@@ -1,24 +1,29 @@
-// Ensembles: many independent heat problems of uneven sizes, advanced by a
-// persistent kernel over a device-side work list, one launch per step.
+// Ensembles: many independent heat problems of uneven sizes, advanced by
+// persistent kernels over device-side work lists, one group of launches per
+// step (or per pass of kblock steps).
 //
 // Each member is cut into tiles (x segments of 64 lanes, y strips, z
 // chunks of `zchunk` planes: the z-march depth).  A work list holds one
-// int4 {member, tx, ty, tz} per tile, costliest first, and a fixed grid of
-// 256-thread workgroups -- as many as are resident on the device at once --
-// walks it with a grid stride.  FP64 members run the FP64 tile (2 rows per
-// lane, 4 waves: 64 x 8 points per z step), bit-identical to h3d_step_f64.
-// FP32 members run the FP32 pass tile with the FP16 residual field (mode
-// H3D_F32_COMP16), so the compensation carries across passes; a pass fuses
-// `kblock` steps (the temporal-blocking depth), bit-identical to
-// h3d_pass_f32 with the same nsteps, rows 4, waves 4.  Inside this
-// persistent kernel the LDS tile measured 8% faster than the
-// wave-independent single-step tile, so it is used for kblock = 1 too.
+// int4 {member, tx, ty, tz} per tile, longest chunks first, and a fixed
+// grid of workgroups -- as many as are resident on the device at once --
+// walks it with a grid stride.  FP64 and FP32 members have a list and a
+// kernel each, launched one after the other, so each kernel runs at the
+// occupancy of its own tile:
+//  - FP64 members: 512-thread workgroups of 16 rows (8 waves x 2 rows),
+//    the FP64 step tile for single steps (bit-identical to h3d_step_f64)
+//    and the FP64 pass tile for passes (bit-identical to h3d_pass_f64);
+//  - FP32 members: 256-thread workgroups of 16 rows (4 waves x 4 rows),
+//    the FP32 pass tile with the FP16 residual field (mode
+//    H3D_F32_COMP16), so the compensation carries across passes; a pass
+//    fuses `kblock` steps, bit-identical to h3d_pass_f32 with the same
+//    nsteps, rows 4, waves 4.  Inside a persistent kernel the LDS tile
+//    measured 8% faster than the wave-independent single-step tile, so it
+//    is used for kblock = 1 too.
 //
-// With kblock = K > 1 a group of K steps is one launch in which every tile
-// takes a K-step pass: FP64 tiles run the FP64 pass tile (4 rows per lane,
-// 4 waves), bit-identical to h3d_pass_f64.  Steps beyond the last full
-// group are single-step launches over every tile.  Every member flips its
-// ping-pong buffers once per launch (h3d_ensemble_where).  The launches of
+// With kblock = K > 1 a group of K steps is one launch per precision in
+// which every tile takes a K-step pass.  Steps beyond the last full group
+// are single-step launches over every tile.  Every member flips its
+// ping-pong buffers once per group (h3d_ensemble_where).  The launches of
 // a run are captured into one hipGraph on request.
 #include <vector>
 #include <algorithm>
@@ -27,12 +32,19 @@
 
 namespace {
 
-constexpr int ENS_WAVES = 4;    // 256-thread workgroups
-constexpr int ENS_F64_ROWS = 2;
-constexpr int ENS_F32_RY = 4;
+// FP64 and FP32 members run in separate persistent kernels, each with the
+// workgroup shape and LDS of its own tile, so neither is held to the
+// other's occupancy.
+constexpr int ENS64_WAVES = 8;  // FP64 tiles: 8 waves x 2 rows = 16 rows
+constexpr int ENS64_ROWS = 2;   // rows per lane of the FP64 step tile
+constexpr int ENS64_PRY = 2;    // rows per lane of the FP64 pass tile
+constexpr int ENS32_WAVES = 4;  // FP32 tiles: 4 waves x 4 rows = 16 rows
+constexpr int ENS32_RY = 4;
 constexpr int ENS_ZCHUNK = 64;  // default z-march depth
 constexpr int ENS_MAXKB = 3;    // deepest pass fused inside the ensemble
-constexpr int ENS_F64_PRY = 4;  // rows per lane of the FP64 pass tile
+
+template <int PREC>
+constexpr int ens_waves() { return PREC ? ENS32_WAVES : ENS64_WAVES; }
 
 struct EMember {
   void *ua, *ub;
@@ -45,29 +57,30 @@ struct EMember {
   int precision;
 };
 
+// Tiles of members of precision PREC (0 FP64, 1 FP32) at pass depth KB;
 // parity: which buffer every member reads this launch.
-template <int KB>
-__global__ void __launch_bounds__(ENS_WAVES * 64)
+template <int KB, int PREC>
+__global__ void __launch_bounds__(ens_waves<PREC>() * 64)
 ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
               int nwork, int parity, int zchunk) {
   extern __shared__ float lds[];
   for (int t = blockIdx.x; t < nwork; t += gridDim.x) {
     const int4 d = work[t];
     const EMember m = mem[d.x];
-    if (m.precision == 0) {
+    if constexpr (PREC == 0) {
       const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
       double *out = static_cast<double *>(parity ? m.ua : m.ub);
       if constexpr (KB > 1) {
         const int zb = 1 + d.w * zchunk;
-        pass_f64_tile<KB, ENS_F64_PRY, ENS_WAVES, false>(
+        pass_f64_tile<KB, ENS64_PRY, ENS64_WAVES, false>(
             reinterpret_cast<double *>(lds), out, in, nullptr,
             static_cast<const double *>(m.kap), m.nx, m.ny, m.nz, m.sy, m.sz, m.factor, zb,
             min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
       } else {
-        march_f64_tile<ENS_F64_ROWS, false>(out, in, nullptr,
-                                            static_cast<const double *>(m.kap),
-                                            m.nx, m.ny, m.sy, m.sz, m.factor, 1,
-                                            m.nz - 2, zchunk, 1, d.y, d.z, d.w);
+        march_f64_tile<ENS64_ROWS, false>(out, in, nullptr,
+                                          static_cast<const double *>(m.kap),
+                                          m.nx, m.ny, m.sy, m.sz, m.factor, 1,
+                                          m.nz - 2, zchunk, 1, d.y, d.z, d.w);
       }
     } else {
       const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
@@ -75,7 +88,7 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
       const unsigned short *cin = parity ? m.cb : m.ca;
       unsigned short *cout = parity ? m.ca : m.cb;
       const int zb = 1 + d.w * zchunk;
-      pass_f32_tile<KB, ENS_F32_RY, ENS_WAVES, 2, false>(
+      pass_f32_tile<KB, ENS32_RY, ENS32_WAVES, 2, false>(
           lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
           m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
           min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
@@ -85,12 +98,13 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
   }
 }
 
-template <int KB>
+template <int KB, int PREC>
 constexpr size_t ens_lds() {
-  const size_t f32 = sizeof(float) * (size_t)pass_lds_floats(KB, ENS_F32_RY, ENS_WAVES);
-  const size_t f64 =
-      KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS_F64_PRY, ENS_WAVES) : 0;
-  return f32 > f64 ? f32 : f64;
+  if constexpr (PREC == 1)
+    return sizeof(float) * (size_t)pass_lds_floats(KB, ENS32_RY, ENS32_WAVES);
+  else
+    return KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS64_PRY, ENS64_WAVES)
+                  : 0;
 }
 
 }  // namespace
@@ -99,7 +113,7 @@ struct h3d_ensemble {
   int device;
   int count;
   int zchunk;
-  int kb;           // FP32 steps per pass
+  int kb;           // steps per pass
   long long done;   // steps taken so far
   long long passes; // passes taken so far (every member flips once per pass)
   EMember *d_mem;
@@ -108,12 +122,12 @@ struct h3d_ensemble {
     int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision;
   };
   std::vector<Tiles> tiles;  // per member
-  // Work lists in one device buffer: every tile at depth kb, every tile at
-  // depth 1 (single-step launches).
+  // Work lists in one device buffer, per list (0: every tile at depth kb,
+  // 1: every tile at depth 1, for the steps past the last full group) and
+  // precision, with the persistent grid of each kernel.
   int4 *d_work;
   int work_cap;
-  int off_kb, n_kb, off_1, n_1;
-  int blocks_kb, blocks_1;  // persistent grid of each kernel instance
+  int off[2][2], n[2][2], blocks[2][2];
   std::vector<unsigned short *> comp;  // owned FP16 residual fields
   // cached graph of `graph_steps` steps starting at buffer parity graph_p
   hipGraphExec_t graph = nullptr;
@@ -141,54 +155,50 @@ static void ensemble_free(h3d_ensemble *e) {
 
 // Kernels whose LDS exceeds the 64 KB default need the attribute raised on
 // each device before they are sized or launched.
-template <int KB>
+template <int KB, int PREC>
 static int allow_lds() {
-  if (ens_lds<KB>() <= 64 * 1024) return H3D_OK;
+  if (ens_lds<KB, PREC>() <= 64 * 1024) return H3D_OK;
   static std::atomic<unsigned long long> done{0};
-  return h3d_allow_big_lds((const void *)ensemble_step<KB>, &done);
+  return h3d_allow_big_lds((const void *)ensemble_step<KB, PREC>, &done);
 }
 
-static int allow_lds_kb(int kb) {
-  return kb == 1 ? allow_lds<1>() : kb == 2 ? allow_lds<2>() : allow_lds<3>();
-}
-
-static int occupancy_blocks(int device, int kb, int nwork, int *blocks) {
-  int cus = 0, per_cu = 0;
-  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-  const int lrc = allow_lds_kb(kb);
+// Persistent grid of the (depth, precision) kernel for nwork tiles: as
+// many workgroups as are resident on the device at once.
+template <int KB, int PREC>
+static int occupancy_one(int cus, int nwork, int *blocks) {
+  const int lrc = allow_lds<KB, PREC>();
   if (lrc != H3D_OK) return lrc;
-  hipError_t he;
-  switch (kb) {
-    case 1: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void *)ensemble_step<1>, ENS_WAVES * 64, ens_lds<1>()); break;
-    case 2: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void *)ensemble_step<2>, ENS_WAVES * 64, ens_lds<2>()); break;
-    default: he = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, (const void *)ensemble_step<3>, ENS_WAVES * 64, ens_lds<3>()); break;
-  }
-  H3D_TRY(he);
+  int per_cu = 0;
+  H3D_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+      &per_cu, (const void *)ensemble_step<KB, PREC>, ens_waves<PREC>() * 64,
+      ens_lds<KB, PREC>()));
   *blocks = std::max(1, std::min(std::max(nwork, 1), cus * std::max(per_cu, 1)));
   return H3D_OK;
 }
 
-// (Re)builds the three work lists for z chunks of `zchunk` planes and FP32
-// passes of `kb` steps, costliest tiles first (stable, so a member's tiles
-// stay together), and sizes the persistent grids.  Drops any cached graph.
+template <int PREC>
+static int occupancy_blocks(int cus, int kb, int nwork, int *blocks) {
+  return kb == 1   ? occupancy_one<1, PREC>(cus, nwork, blocks)
+         : kb == 2 ? occupancy_one<2, PREC>(cus, nwork, blocks)
+                   : occupancy_one<3, PREC>(cus, nwork, blocks);
+}
+
+// (Re)builds the work lists for z chunks of `zchunk` planes and passes of
+// `kb` steps, costliest tiles first (stable, so a member's tiles stay
+// together), and sizes the persistent grids.  Drops any cached graph.
 static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   if (zchunk < 1 || kb < 1 || kb > ENS_MAXKB) return H3D_EARG;
   struct Job { long long cost; int4 d; };
-  auto list = [&](int depth) {
+  auto list = [&](int depth, int prec) {
     std::vector<Job> jobs;
     for (int m = 0; m < e->count; ++m) {
       const h3d_ensemble::Tiles &t = e->tiles[m];
+      if (t.precision != prec) continue;
       const int tx_n = t.tiles_x[depth], ty_n = t.tiles_y[depth];
       const int chunks = (t.planes + zchunk - 1) / zchunk;
       for (int tz = 0; tz < chunks; ++tz) {
         const int planes = std::min(zchunk, t.planes - tz * zchunk);
-        // relative cost per plane: FP64 steps 3, FP64 passes 2 + depth,
-        // FP32 passes 2 * depth
-        const long long cost =
-            (long long)planes * (t.precision ? 2 * depth : depth > 1 ? 2 + depth : 3);
+        const long long cost = planes;  // one tile's planes
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
       }
@@ -199,14 +209,29 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
     for (size_t j = 0; j < jobs.size(); ++j) w[j] = jobs[j].d;
     return w;
   };
-  std::vector<int4> wkb = list(kb);
-  std::vector<int4> w1 = kb > 1 ? list(1) : std::vector<int4>();
-  const size_t total = wkb.size() + w1.size();
+  std::vector<int4> lists[2][2];
+  for (int prec = 0; prec < 2; ++prec) {
+    lists[0][prec] = list(kb, prec);
+    if (kb > 1) lists[1][prec] = list(1, prec);
+  }
+  size_t total = 0;
+  for (auto &l : lists)
+    for (auto &w : l) total += w.size();
   if (total > 0x7fffffffu) return H3D_ELIMIT;
   std::vector<int4> all;
   all.reserve(total);
-  all.insert(all.end(), wkb.begin(), wkb.end());
-  all.insert(all.end(), w1.begin(), w1.end());
+  int off[2][2], n[2][2];
+  for (int li = 0; li < 2; ++li)
+    for (int prec = 0; prec < 2; ++prec) {
+      off[li][prec] = (int)all.size();
+      n[li][prec] = (int)lists[li][prec].size();
+      all.insert(all.end(), lists[li][prec].begin(), lists[li][prec].end());
+    }
+  if (kb == 1)  // the single-step lists are the kb lists themselves
+    for (int prec = 0; prec < 2; ++prec) {
+      off[1][prec] = off[0][prec];
+      n[1][prec] = n[0][prec];
+    }
   if ((int)total > e->work_cap) {
     if (e->d_work) (void)hipFree(e->d_work);
     e->d_work = nullptr;
@@ -216,45 +241,51 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   }
   if (total)
     H3D_TRY(hipMemcpy(e->d_work, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
-  e->off_kb = 0;
-  e->n_kb = (int)wkb.size();
-  // with kb = 1 the single-step list is the kb list itself
-  e->off_1 = kb > 1 ? e->n_kb : 0;
-  e->n_1 = kb > 1 ? (int)w1.size() : e->n_kb;
-  int rc = occupancy_blocks(e->device, kb, e->n_kb, &e->blocks_kb);
-  if (rc == H3D_OK) rc = occupancy_blocks(e->device, 1, e->n_1, &e->blocks_1);
-  if (rc != H3D_OK) return rc;
+  int cus = 0;
+  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+  for (int li = 0; li < 2; ++li) {
+    const int depth = li == 0 ? kb : 1;
+    int rc = occupancy_blocks<0>(cus, depth, n[li][0], &e->blocks[li][0]);
+    if (rc == H3D_OK) rc = occupancy_blocks<1>(cus, depth, n[li][1], &e->blocks[li][1]);
+    if (rc != H3D_OK) return rc;
+    for (int prec = 0; prec < 2; ++prec) {
+      e->off[li][prec] = off[li][prec];
+      e->n[li][prec] = n[li][prec];
+    }
+  }
   e->zchunk = zchunk;
   e->kb = kb;
   drop_graph(e);
   return H3D_OK;
 }
 
-// One launch over list 0 (every tile, passes of kb steps) or 1 (every
-// tile, single steps).
-static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
-  const int4 *w = e->d_work + (list == 0 ? e->off_kb : e->off_1);
-  const int n = list == 0 ? e->n_kb : e->n_1;
+template <int KB, int PREC>
+static int launch_one(h3d_ensemble *e, int li, int parity, hipStream_t s) {
+  const int n = e->n[li][PREC];
   if (n == 0) return H3D_OK;
-  const int depth = list == 0 ? e->kb : 1;
-  const int blocks = list == 0 ? e->blocks_kb : e->blocks_1;
-  const int lrc = allow_lds_kb(depth);
+  const int lrc = allow_lds<KB, PREC>();
   if (lrc != H3D_OK) return lrc;
-  switch (depth) {
-    case 1:
-      hipLaunchKernelGGL(ensemble_step<1>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<1>(),
-                         s, e->d_mem, w, n, parity, e->zchunk);
-      break;
-    case 2:
-      hipLaunchKernelGGL(ensemble_step<2>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<2>(),
-                         s, e->d_mem, w, n, parity, e->zchunk);
-      break;
-    default:
-      hipLaunchKernelGGL(ensemble_step<3>, dim3(blocks), dim3(ENS_WAVES * 64), ens_lds<3>(),
-                         s, e->d_mem, w, n, parity, e->zchunk);
-      break;
-  }
+  constexpr size_t lds = ens_lds<KB, PREC>();
+  constexpr unsigned threads = ens_waves<PREC>() * 64;
+  auto fn = ensemble_step<KB, PREC>;
+  hipLaunchKernelGGL(fn, dim3(e->blocks[li][PREC]), dim3(threads), lds, s, e->d_mem,
+                     e->d_work + e->off[li][PREC], n, parity, e->zchunk);
   return h3d_launch_status();
+}
+
+template <int KB>
+static int launch_both(h3d_ensemble *e, int li, int parity, hipStream_t s) {
+  const int rc = launch_one<KB, 0>(e, li, parity, s);
+  return rc != H3D_OK ? rc : launch_one<KB, 1>(e, li, parity, s);
+}
+
+// One group: the FP64 and the FP32 kernel over list 0 (every tile, passes
+// of kb steps) or list 1 (every tile, single steps).
+static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
+  const int depth = list == 0 ? e->kb : 1;
+  return depth == 1   ? launch_both<1>(e, list, parity, s)
+         : depth == 2 ? launch_both<2>(e, list, parity, s)
+                      : launch_both<3>(e, list, parity, s);
 }
 
 // Passes (launches) in a run of `steps` steps: full groups plus single
@@ -314,9 +345,9 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
       // pass tile (FP32 at any depth, FP64 from depth 2) keeps
       // (64 - 2d) x (RY x waves - 2d) of its cells
       const bool step64 = x.precision == 0 && d == 1;
-      const int ry = x.precision ? ENS_F32_RY : ENS_F64_PRY;
+      const int rows = x.precision ? ENS32_RY * ENS32_WAVES : ENS64_PRY * ENS64_WAVES;
       const int OX = step64 ? 64 : 64 - 2 * d;
-      const int OY = step64 ? ENS_F64_ROWS * ENS_WAVES : ry * ENS_WAVES - 2 * d;
+      const int OY = step64 ? ENS64_ROWS * ENS64_WAVES : rows - 2 * d;
       t.tiles_x[d] = (x.g.nx - 2 + OX - 1) / OX;
       t.tiles_y[d] = (x.g.ny - 2 + OY - 1) / OY;
     }
@@ -436,7 +467,7 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
   if (rc != H3D_OK) return rc;
   if (out) {
     out->rows = 0;
-    out->waves = ENS_WAVES;
+    out->waves = 0;  // each precision runs its own workgroup shape
     out->zchunk = best_zc;
     out->kblock = best_kb;
   }
@@ -452,8 +483,8 @@ extern "C" int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b) 
 extern "C" int h3d_ensemble_info(const h3d_ensemble *e, int *nwork,
                                  int *nblocks, long long *steps_done) {
   if (!e) return H3D_EARG;
-  if (nwork) *nwork = e->n_kb;
-  if (nblocks) *nblocks = e->blocks_kb;
+  if (nwork) *nwork = e->n[0][0] + e->n[0][1];
+  if (nblocks) *nblocks = e->blocks[0][0] + e->blocks[0][1];
   if (steps_done) *steps_done = e->done;
   return H3D_OK;
 }
