@@ -286,8 +286,8 @@ typedef struct h3d_member {
 typedef struct h3d_ensemble h3d_ensemble;
 
 /* Builds the tile work lists on `device_id` and allocates the FP32
- * members' residual fields (zeroed).  Tiles are fixed in x and y (16 rows;
- * FP64 members in 512-thread and FP32 members in 256-thread workgroups,
+ * members' residual fields (zeroed).  Tiles are fixed in x and y (32 rows;
+ * FP64 members in 1024-thread and FP32 members in 512-thread workgroups,
  * each precision its own persistent kernel).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
  * depth (planes per work item, default 64) and tune->kblock the
  * temporal-blocking depth (steps fused per pass of every member, 1..3,
