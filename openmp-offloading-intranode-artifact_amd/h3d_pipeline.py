@@ -12,12 +12,21 @@ artifact's MI250X variants, and aggregates run logs from either side.
 
   python h3d_pipeline.py run [--sizes 512 640 ...] [--reps 3] [--out DIR]
   python h3d_pipeline.py aggregate results_*/run.log [-o summary.csv]
+  python h3d_pipeline.py gamma summary.csv [--exe 1-hip-h3d] [--output F]
+
+`gamma` is the machine constant of the artifact's performance model
+(3D_Heat_Benchmark/*/Source-Results/model/compute_gamma.py): per grid of one
+variant, gamma = T1 / (steps (N-2)^3) seconds per point-update, and its
+plateau, the mean over the largest grids.
 
 The run case of size N is the artifact's: `h3d N T` with T = STEPS * dt
 printed %.7g, dt = h^2/(12 kappa_max), h = 1/(N-1), kappa_max = 0.95 and
 STEPS = 500 (submit.sh:10-17,72-75), WRITE_TXT=0 (run.sh:52).
 """
 import argparse
+import csv
+import io
+import math
 import os
 import re
 import subprocess
@@ -142,6 +151,48 @@ def summarize(records):
     return [SUMMARY_HEADER] + rows
 
 
+GAMMA_FIELDS = ("N", "interior_points", "timesteps", "T1_seconds", "gamma_s_per_point")
+
+
+def gamma_rows(summary_lines, variant="1-omp", time_column="median_sec"):
+    """Machine constant per grid from summary.csv lines, as
+    compute_gamma.py's read_summary (:124-186): rows of `variant` with an
+    integer N > 2, steps > 0 and a finite positive `time_column`; gamma =
+    T1 / (steps (N-2)^3); sorted by N.  Other rows are skipped."""
+    out = []
+    for row in csv.DictReader(summary_lines):
+        if (row.get("variant") or "").strip() != variant:
+            continue
+        try:
+            n, steps = int(row["N"]), int(row["steps"])
+            t1 = float(row[time_column])
+        except (KeyError, TypeError, ValueError):
+            continue
+        if n <= 2 or steps <= 0 or not math.isfinite(t1) or t1 <= 0:
+            continue
+        interior = (n - 2) ** 3
+        out.append({"N": n, "interior_points": interior, "timesteps": steps,
+                    "T1_seconds": t1, "gamma_s_per_point": t1 / (steps * interior)})
+    out.sort(key=lambda r: r["N"])
+    return out
+
+
+def gamma_csv(rows):
+    """gamma_single_gpu.csv text of gamma rows (compute_gamma.py:203-219:
+    the csv module's writer, CRLF lines, numbers as Python prints them)."""
+    buf = io.StringIO(newline="")
+    w = csv.DictWriter(buf, fieldnames=GAMMA_FIELDS)
+    w.writeheader()
+    w.writerows(rows)
+    return buf.getvalue()
+
+
+def gamma_plateau(rows, tail=3):
+    """Mean gamma of the `tail` largest grids (compute_gamma.py:314-318)."""
+    last = rows[-min(tail, len(rows)):]
+    return sum(r["gamma_s_per_point"] for r in last) / len(last)
+
+
 def run_case(binary, variant, n, steps=500, rep=1, job_id="local", timeout=1800,
              extra_args=(), gpus=1, mask="0"):
     """One timed run of `binary n T`, as run.sh's run_bench (:196-232).
@@ -208,6 +259,28 @@ def cmd_aggregate(a):
         sys.stdout.write(summary)
 
 
+def cmd_gamma(a):
+    with open(a.summary, newline="", encoding="utf-8") as f:
+        rows = gamma_rows(f, a.exe, a.time_column)
+    if not rows:
+        sys.exit(f"No usable rows found for variant {a.exe!r} in {a.summary}")
+    print(f"{'N':>6} {'interior=(n-2)^3':>18} {'steps':>7} {'T1 [s]':>12} "
+          f"{'gamma [s/pt]':>15} {'gamma [ps/pt]':>15}")
+    for r in rows:
+        g = r["gamma_s_per_point"]
+        print(f"{r['N']:>6} {r['interior_points']:>18} {r['timesteps']:>7} "
+              f"{r['T1_seconds']:>12.6f} {g:>15.4e} {g * 1e12:>15.3f}")
+    last = rows[-min(a.tail, len(rows)):]
+    p = gamma_plateau(rows, a.tail)
+    print(f"Plateau gamma (mean of N = {', '.join(str(r['N']) for r in last)}): "
+          f"{p:.4e} s/point-update ({p * 1e12:.2f} ps/point-update)")
+    out = a.output or os.path.join(os.path.dirname(os.path.abspath(a.summary)),
+                                   "gamma_single_gpu.csv")
+    with open(out, "w", newline="", encoding="utf-8") as f:
+        f.write(gamma_csv(rows))
+    print(f"Wrote {out}")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -226,8 +299,19 @@ def main(argv=None):
     g = sub.add_parser("aggregate", help="summary.csv from run logs")
     g.add_argument("logs", nargs="+")
     g.add_argument("-o", default="")
+    m = sub.add_parser("gamma", help="machine constant from a summary.csv")
+    m.add_argument("summary")
+    m.add_argument("--exe", default="1-omp", help="variant to select (default: 1-omp)")
+    m.add_argument("--time-column", default="median_sec",
+                   choices=("min_sec", "median_sec", "max_sec"))
+    m.add_argument("--tail", type=int, default=3,
+                   help="number of largest grids in the plateau mean")
+    m.add_argument("--output", default="",
+                   help="default: gamma_single_gpu.csv beside the summary")
     a = ap.parse_args(argv)
-    (cmd_run if a.cmd == "run" else cmd_aggregate)(a)
+    if a.cmd == "gamma" and a.tail <= 0:
+        ap.error("--tail must be positive")
+    {"run": cmd_run, "aggregate": cmd_aggregate, "gamma": cmd_gamma}[a.cmd](a)
 
 
 if __name__ == "__main__":

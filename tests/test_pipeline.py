@@ -1,7 +1,10 @@
 """Timing rows and summaries in the artifact's pipeline formats (run.sh,
-aggregate.sh), checked against the reference's own recorded MI250X HIP
-results: the aggregator must rebuild its summary.csv byte for byte from its
-run logs (tests/golden/make_pipeline_fixture.py wrote the fixtures)."""
+aggregate.sh, model/compute_gamma.py), checked against the reference's own
+recorded results: the aggregator must rebuild its MI250X HIP summary.csv byte
+for byte from its run logs, and the machine constant its gamma_single_gpu.csv
+files from their summaries (tests/golden/make_pipeline_fixture.py wrote the
+fixtures)."""
+import io
 import os
 import stat
 import sys
@@ -116,3 +119,54 @@ def test_run_and_aggregate_commands(tmp_path):
                  "1-hip-h3d,1,128,2,0,1.280000,1.280000,1.280000,0.0,129"]
     P.main(["aggregate", str(out / "run.log"), "-o", str(tmp_path / "s2.csv")])
     assert (tmp_path / "s2.csv").read_text().splitlines() == s
+
+
+@pytest.mark.parametrize("machine", ["mi250x", "h100", "max1550"])
+def test_gamma_rebuilds_reference_files(machine):
+    """compute_gamma.py's gamma_single_gpu.csv for each machine, byte for
+    byte, from the OpenMP summary.csv it was computed from."""
+    with open(os.path.join(GOLDEN, f"gamma_{machine}_summary.csv"), newline="") as f:
+        rows = P.gamma_rows(f)
+    with open(os.path.join(GOLDEN, f"gamma_{machine}_single_gpu.csv"), "rb") as f:
+        want = f.read()
+    assert P.gamma_csv(rows).encode() == want
+    assert [r["N"] for r in rows] == list(P.N_LIST)
+
+
+def test_gamma_plateau_and_skip_rules():
+    # the plateau is the mean of the three largest grids' recorded gammas
+    with open(os.path.join(GOLDEN, "gamma_mi250x_single_gpu.csv")) as f:
+        recorded = [float(line.split(",")[4]) for line in f.read().splitlines()[1:]]
+    with open(os.path.join(GOLDEN, "gamma_mi250x_summary.csv"), newline="") as f:
+        rows = P.gamma_rows(f)
+    assert P.gamma_plateau(rows) == sum(recorded[-3:]) / 3
+    assert P.gamma_plateau(rows, tail=50) == sum(recorded) / len(recorded)
+    text = "\n".join([
+        P.SUMMARY_HEADER,
+        " 1-hip-h3d ,1,66,3,0,1.0,2.0,3.0,0.0,10",  # variant compared stripped
+        "1-hip-h3d,1,2,3,0,1.0,2.0,3.0,0.0,10",    # N <= 2
+        "1-hip-h3d,1,34,3,0,1.0,2.0,3.0,0.0,0",    # no steps
+        "1-hip-h3d,1,34,3,0,1.0,nan,3.0,0.0,10",   # non-finite time
+        "1-hip-h3d,1,34,3,0,1.0,0,3.0,0.0,10",     # zero time
+        "1-hip-h3d,1,34,3,0,1.0,x,3.0,0.0,10",     # unparsable
+        "1-hip,1,34,3,0,1.0,2.0,3.0,0.0,10",       # another variant
+        "1-hip-h3d,1,18,3,0,1.0,0.5,3.0,0.0,4",
+    ]) + "\n"
+    rows = P.gamma_rows(io.StringIO(text), "1-hip-h3d")
+    assert [(r["N"], r["timesteps"], r["T1_seconds"]) for r in rows] == [(18, 4, 0.5),
+                                                                      (66, 10, 2.0)]
+    assert rows[0]["gamma_s_per_point"] == 0.5 / (4 * 16 ** 3)
+    assert P.gamma_rows(io.StringIO(text), "1-hip-h3d", "max_sec")[1]["T1_seconds"] == 3.0
+
+
+def test_gamma_command(tmp_path, capsys):
+    src = tmp_path / "summary.csv"
+    src.write_text(P.SUMMARY_HEADER + "\n1-hip-h3d,1,512,3,0,0.16,0.164464,0.17,0.2,501\n")
+    out = tmp_path / "g.csv"
+    P.main(["gamma", str(src), "--exe", "1-hip-h3d", "--output", str(out)])
+    assert out.read_bytes() == (
+        "N,interior_points,timesteps,T1_seconds,gamma_s_per_point\r\n"
+        f"512,132651000,501,0.164464,{0.164464 / (501 * 132651000)!r}\r\n").encode()
+    assert "Plateau gamma (mean of N = 512)" in capsys.readouterr().out
+    with pytest.raises(SystemExit):
+        P.main(["gamma", str(src), "--exe", "1-omp"])
