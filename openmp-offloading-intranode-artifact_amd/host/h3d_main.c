@@ -32,7 +32,9 @@
  *   --graph            replay the step loop from a hipGraph
  *   --slab P:p         run slab p of a P-way z split (P = 2 or 4) with its
  *                      two ghost planes refreshed every step by in-HBM face
- *                      pack/unpack from the slab's own edge planes
+ *                      pack/unpack from the slab's own edge planes; with
+ *                      --kblock K, K ghost planes per side refreshed by a
+ *                      band of K planes after every K-step fused pass
  *   --check            FP32 runs: also run the FP64 kernel on the same
  *                      inputs and report rel-L2 / max errors against it
  *   --device D         HIP device (default 0)
@@ -172,8 +174,8 @@ static opts parse(int argc, char **argv)
     fprintf(stderr, "ERROR: --slab runs the FP64 kernel only\n");
     exit(1);
   }
-  if (o.slab_parts && o.tune.kblock > 1) {
-    fprintf(stderr, "ERROR: --slab runs single steps (no --kblock)\n");
+  if (o.slab_parts && o.tune.kblock > 4) {
+    fprintf(stderr, "ERROR: --slab --kblock K: K ghost planes per side, 1..4\n");
     exit(1);
   }
   return o;
@@ -247,6 +249,9 @@ int main(int argc, char **argv)
 
   /* ---- slab geometry -------------------------------------------------- */
   int nz = n, gk0 = 0; /* planes held and first held global plane */
+  /* a slab holds `ghost` ghost planes per side and advances its owned
+   * planes `ghost` steps per fused pass between band exchanges */
+  const int ghost = o.slab_parts ? (o.tune.kblock > 1 ? o.tune.kblock : 1) : 0;
   if (o.slab_parts) {
     int nz_d[4], gk_start[4];
     if (o.slab_index < 0 || o.slab_index >= o.slab_parts ||
@@ -258,8 +263,14 @@ int main(int argc, char **argv)
     printf("Block sizes:");
     for (int p = 0; p < o.slab_parts; p++) printf(" nz%d=%d", p, nz_d[p]);
     printf("\n");
-    nz = nz_d[o.slab_index];
-    gk0 = gk_start[o.slab_index] - 1; /* ghost plane 0 is global gk_start-1 */
+    nz = nz_d[o.slab_index] - 2 + 2 * ghost;
+    gk0 = gk_start[o.slab_index] - ghost; /* first ghost plane */
+    if (nz < 3 * ghost) {
+      fprintf(stderr, "ERROR: a slab of %d owned planes cannot hold %d ghost planes per side\n",
+              nz_d[o.slab_index] - 2, ghost);
+      return 1;
+    }
+    if (ghost > 1) printf("Ghost planes per side: %d (band exchange every %d steps)\n", ghost, ghost);
   }
   const h3d_grid g = {n, n, nz, n, (long long)n * n}; /* host fields */
   const int sy = pitch_of(n);
@@ -282,11 +293,24 @@ int main(int argc, char **argv)
     kappa_max = 0.95;
     rhs = (double *)xmalloc(cells * sizeof(double));
     if (o.slab_parts)
-      CHECK(h3d_init_partition(u0, rhs, kap, nz, gk0 + 1, 1, n, n, n, h, dt));
+      CHECK(h3d_init_partition(u0, rhs, kap, nz, gk0 + ghost, ghost, n, n, n, h, dt));
     else
       CHECK(h3d_init_reference(u0, rhs, kap, &g));
   } else {
-    CHECK(h3d_gen_seeded(u0, kap, &g, gk0, n, n, n, o.kappa_kind, o.seed, 0));
+    /* planes of a deep slab beyond the global domain (first and last slab)
+     * repeat the nearest domain plane; they are ghosts, which the first
+     * band exchange overwrites */
+    const int lo = gk0 < 0 ? 0 : gk0, hi = gk0 + nz > n ? n : gk0 + nz;
+    const size_t plane = (size_t)n * n;
+    const h3d_grid gin = {n, n, hi - lo, n, (long long)plane};
+    CHECK(h3d_gen_seeded(u0 + (size_t)(lo - gk0) * plane, kap + (size_t)(lo - gk0) * plane,
+                         &gin, lo, n, n, n, o.kappa_kind, o.seed, 0));
+    for (int k = 0; k < nz; k++) {
+      const int src = k < lo - gk0 ? lo - gk0 : (k >= hi - gk0 ? hi - gk0 - 1 : k);
+      if (src == k) continue;
+      memcpy(u0 + (size_t)k * plane, u0 + (size_t)src * plane, plane * sizeof(double));
+      memcpy(kap + (size_t)k * plane, kap + (size_t)src * plane, plane * sizeof(double));
+    }
     h = 1.0 / (n - 1);
     double fmax;
     if (o.slab_parts) {
@@ -411,11 +435,11 @@ int main(int argc, char **argv)
   }
   if (o.autotune) {
     h3d_tune t;
-    /* FP64: mode 1 tunes shapes that hold rhs; the slab step runs single
-     * steps */
+    /* FP64: mode 1 tunes shapes that hold rhs; a slab runs passes of as
+     * many steps as it has ghost planes per side */
     CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &gd,
                        f32 ? mode : (rhs ? 1 : 0),
-                       o.slab_parts ? 1 : o.tune.kblock, stream, &t));
+                       o.slab_parts ? ghost : o.tune.kblock, stream, &t));
     tune = t;
     printf("Autotuned: rows=%d waves=%d zchunk=%d kblock=%d\n", tune.rows,
            tune.waves, tune.zchunk, tune.kblock);
@@ -424,8 +448,8 @@ int main(int argc, char **argv)
   /* slab: face buffers for the in-HBM exchange */
   double *d_face_lo = NULL, *d_face_hi = NULL;
   if (o.slab_parts) {
-    CHECK(h3d_target_alloc((void **)&d_face_lo, (size_t)n * n * 8, o.device));
-    CHECK(h3d_target_alloc((void **)&d_face_hi, (size_t)n * n * 8, o.device));
+    CHECK(h3d_target_alloc((void **)&d_face_lo, (size_t)ghost * n * n * 8, o.device));
+    CHECK(h3d_target_alloc((void **)&d_face_hi, (size_t)ghost * n * n * 8, o.device));
   }
 
   printf("\n=== SIMULATION STARTS ===\n");
@@ -436,9 +460,14 @@ int main(int argc, char **argv)
   if (!f32 && o.slab_parts) {
     /* middle-slab shape (2-openmp.c:25-55): boundary planes and the face
      * exchange overlapped with the interior on a second stream */
-    CHECK(h3d_run_slab_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, d_face_lo,
-                           d_face_hi, steps, &tune, o.graph, stream, &dev_ms,
-                           &in_b));
+    if (ghost == 1)
+      CHECK(h3d_run_slab_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, d_face_lo,
+                             d_face_hi, steps, &tune, o.graph, stream, &dev_ms,
+                             &in_b));
+    else /* K ghost planes: K-step passes, one band exchange per pass */
+      CHECK(h3d_run_slab_pass_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, ghost,
+                                  d_face_lo, d_face_hi, steps, &tune, o.graph,
+                                  stream, &dev_ms, &in_b));
   } else if (!f32 && (o.graph || tune.kblock > 1)) {
     /* replayed from a hipGraph, or passes of tune.kblock fused steps */
     CHECK(h3d_run_f64(d_ua, d_ub, d_rhs, d_kap, &gd, factor, steps, &tune, o.graph,
@@ -463,10 +492,10 @@ int main(int argc, char **argv)
   printf("Time = %f seconds\n", timer);
   printf("=== SIMULATION ENDS ===\n");
 
-  const double lups = (double)(n - 2) * (n - 2) * (nz - 2) * steps;
+  const double lups = (double)(n - 2) * (n - 2) * (nz - 2 * (ghost ? ghost : 1)) * steps;
   double bpl = f32 ? (o.bf16 ? 10.0 : 12.0) : (rhs ? 32.0 : 24.0);
   if (o.prec == 3) bpl += 4.0;
-  if (!o.slab_parts) bpl /= (tune.kblock > 0 ? tune.kblock : 1);
+  bpl /= o.slab_parts ? ghost : (tune.kblock > 0 ? tune.kblock : 1);
   if (timer > 0)
     printf("Performance: %.3f GLUP/s, %.1f GB/s algorithmic (%.2f B/LUP)\n",
            lups / timer / 1e9, lups * bpl / timer / 1e9, bpl);

@@ -89,6 +89,21 @@ def test_cli_slab_runs_with_face_exchange(tmp_path):
     assert np.array_equal(v[0], v[8]) and np.array_equal(v[9], v[1])
 
 
+@pytest.mark.parametrize("mode,part", [("reference", 1), ("seeded", 0)])
+def test_cli_deep_slab_runs_with_band_exchange(tmp_path, mode, part):
+    """--slab with --kblock 2: two ghost planes per side, 2-step passes and
+    one band exchange per pass; the first slab of a seeded run starts one
+    plane below the domain (the stand-in repeats the boundary plane)."""
+    out = run(["34", "--steps", "5", "--mode", mode, "--slab", f"4:{part}", "--kblock", "2"],
+              str(tmp_path))
+    assert "Ghost planes per side: 2" in out and "5 timesteps complete" in out
+    vals = np.loadtxt(tmp_path / f"u_h3d_slab{part}of4.txt")
+    assert vals.size == 12 * 34 * 34 and np.all(np.isfinite(vals))
+    v = vals.reshape(12, 34, 34)
+    # ghosts hold the z-periodic stand-in bands of owned planes
+    assert np.array_equal(v[0:2], v[8:10]) and np.array_equal(v[10:12], v[2:4])
+
+
 def test_cli_autotune(tmp_path):
     out = run(["70", "--steps", "4", "--mode", "seeded", "--autotune"], str(tmp_path),
               write=False)
