@@ -191,6 +191,13 @@ def test_cli_rejects_bad_arguments():
     r = subprocess.run([exe, "9", "--slab", "4:0", "--steps", "1"], capture_output=True,
                        text=True, env=dict(os.environ, WRITE_TXT="0"))
     assert r.returncode == 1 and "cannot be split" in r.stderr
+    # deep slabs: 1..4 ghost planes per side, and at least as many owned planes
+    r = subprocess.run([exe, "34", "--slab", "4:1", "--kblock", "5", "--steps", "1"],
+                       capture_output=True, text=True, env=dict(os.environ, WRITE_TXT="0"))
+    assert r.returncode == 1 and "1..4" in r.stderr
+    r = subprocess.run([exe, "14", "--slab", "4:1", "--kblock", "4", "--steps", "1"],
+                       capture_output=True, text=True, env=dict(os.environ, WRITE_TXT="0"))
+    assert r.returncode == 1 and "cannot hold 4 ghost planes" in r.stderr
 
 
 def test_verify_tool_exit_codes_without_device(tmp_path):
