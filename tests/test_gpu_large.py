@@ -97,9 +97,9 @@ def test_1000_steps_invariants(gpu, big):
 
 
 def test_fp32_compensated_target_accuracy(gpu, big):
-    """The north-star accuracy bar: FP32 storage, compensated, temporally
-    blocked, 1000 steps at 512^3: rel-L2 <= 1e-5 against the FP64 kernel on
-    the same (FP32-rounded) inputs."""
+    """The accuracy target of the FP32 path: FP32 storage, compensated,
+    temporally blocked, 1000 steps at 512^3: rel-L2 <= 1e-5 against the FP64
+    kernel on the same (FP32-rounded) inputs."""
     g = big["g"]
     u32 = big["u"].astype(np.float32)
     k32 = big["k"].astype(np.float32)
