@@ -351,6 +351,22 @@ int h3d_stream_probe(size_t bytes, int reps, int device_id, double *copy_gbs,
 int h3d_transfer_probe(size_t bytes, int reps, int pinned, int device_id,
                        double *out);
 
+/* One host<->device transfer of `bytes` (a multiple of 8, >= 8) timed as
+ * the reference's transfer benchmarks time it, with the data it carried
+ * checked afterwards (Bandwidth_Measurements/AMD-MI250X/Synchronous-test-
+ * H-to-D-Measurements/hip-1.hip:67-120 and its D-to-H twin;
+ * Asynchronous-test-H-to-D-Measurements/hip-1.hip:79-98 for async = 1):
+ * two warm-up copies, then one copy timed by the host clock around a
+ * blocking hipMemcpy (async = 0) or by HIP events around hipMemcpyAsync on
+ * a stream (async = 1).  direction 0 = host to device, 1 = device to host;
+ * pinned != 0: hipHostMalloc'ed host buffer, else malloc'ed (the
+ * reference's).  Every 64-bit word carries its own index-derived value;
+ * all words are checked up to 1 GiB, above that the first and last 4096
+ * and 4096 spread through the buffer.  out[0] GB/s, out[1] seconds,
+ * out[2] words that arrived wrong (0 = verified), out[3] words checked. */
+int h3d_transfer_bench(size_t bytes, int pinned, int async, int direction,
+                       int device_id, double *out);
+
 /* ---------------------------------------------------------------------- */
 /* Host-side problem set-up (plain C; no device involved).                 */
 

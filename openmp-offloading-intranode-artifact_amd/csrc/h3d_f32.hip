@@ -77,14 +77,32 @@ h3d_pass_f32_kernel(float *__restrict__ uout, const float *__restrict__ uin,
                                        min(nz - 2, zb + zchunk - 1), tx, ty);
 }
 
+// Arguments of one FP32 pass launch.  With `occ` set, nothing is launched:
+// the shape's resident workgroups per CU go to *occ and its tiles per z
+// chunk to *tiles (what the z-chunk choice needs to fill whole rounds).
+struct F32Launch {
+  float *uo;
+  const float *ui;
+  const float *kf;
+  const uint16_t *kb;
+  uint16_t *co;
+  const uint16_t *ci;
+  const h3d_grid *g;
+  float factor;
+  int zchunk;
+  hipStream_t s;
+  bool bf16;  // kappa as BF16 bits (kb) rather than FP32 (kf)
+  int *occ;
+  long long *tiles;
+};
+
 template <int KB, int RY, int W, int MODE, bool BF16>
-int launch_one(float *uo, const float *ui, const float *kf, const uint16_t *kb,
-               uint16_t *co, const uint16_t *ci, const h3d_grid *g,
-               float factor, int zchunk, hipStream_t s) {
+int launch_one(const F32Launch &a) {
   constexpr int OX = 64 - 2 * KB, OY = W * RY - 2 * KB;
   if (OY < 1) return H3D_EARG;
   constexpr size_t lds = sizeof(float) * (size_t)pass_lds_floats(KB, RY, W);
   if (lds > 160 * 1024) return H3D_ELIMIT;
+  const h3d_grid *g = a.g;
   const int tiles_x = (g->nx - 2 + OX - 1) / OX;
   const int tiles_y = (g->ny - 2 + OY - 1) / OY;
   auto fn = h3d_pass_f32_kernel<KB, RY, W, MODE, BF16>;
@@ -93,42 +111,72 @@ int launch_one(float *uo, const float *ui, const float *kf, const uint16_t *kb,
     const int big_rc = h3d_allow_big_lds((const void *)fn, &big_lds_done);
     if (big_rc != H3D_OK) return big_rc;
   }
+  if (a.occ) {
+    // resident workgroups per CU depend on the kernel alone: asked once
+    static std::atomic<int> occ_known{0};
+    int occ = occ_known.load(std::memory_order_relaxed);
+    if (occ == 0) {
+      H3D_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * W, lds));
+      occ_known.store(occ, std::memory_order_relaxed);
+    }
+    *a.occ = occ;
+    if (a.tiles) *a.tiles = (long long)tiles_x * tiles_y;
+    return H3D_OK;
+  }
   const long long nb =
-      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+      (long long)tiles_x * tiles_y * ((g->nz - 2 + a.zchunk - 1) / a.zchunk);
   if (nb > 0x7fffffffLL) return H3D_ELIMIT;
-  hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, s, uo, ui, kf,
-                     (const unsigned short *)kb, (unsigned short *)co,
-                     (const unsigned short *)ci, g->nx, g->ny, g->nz, g->sy,
-                     g->sz, factor, zchunk, tiles_x, tiles_y);
+  hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, a.s, a.uo, a.ui, a.kf,
+                     (const unsigned short *)a.kb, (unsigned short *)a.co,
+                     (const unsigned short *)a.ci, g->nx, g->ny, g->nz, g->sy,
+                     g->sz, a.factor, a.zchunk, tiles_x, tiles_y);
   return h3d_launch_status();
 }
 
 template <int KB, int RY, int W>
-int launch_kb_ry_w(float *uo, const float *ui, const float *kf,
-                   const uint16_t *kb, uint16_t *co, const uint16_t *ci,
-                   const h3d_grid *g, float factor, int mode, int zchunk,
-                   hipStream_t s) {
-  const bool bf = kb != nullptr;
-  if (mode == 0)
-    return bf ? launch_one<KB, RY, W, 0, true>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s)
-              : launch_one<KB, RY, W, 0, false>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s);
-  if (mode == 1)
-    return bf ? launch_one<KB, RY, W, 1, true>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s)
-              : launch_one<KB, RY, W, 1, false>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s);
-  return bf ? launch_one<KB, RY, W, 2, true>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s)
-            : launch_one<KB, RY, W, 2, false>(uo, ui, kf, kb, co, ci, g, factor, zchunk, s);
+int launch_kb_ry_w(const F32Launch &a, int mode) {
+  const bool bf = a.bf16;
+  if (mode == 0) return bf ? launch_one<KB, RY, W, 0, true>(a) : launch_one<KB, RY, W, 0, false>(a);
+  if (mode == 1) return bf ? launch_one<KB, RY, W, 1, true>(a) : launch_one<KB, RY, W, 1, false>(a);
+  return bf ? launch_one<KB, RY, W, 2, true>(a) : launch_one<KB, RY, W, 2, false>(a);
 }
 
+// Shapes (rows per lane, waves) offered per depth.  Tiles of 16 waves
+// (1024 threads, one workgroup per CU) keep more of their rows at depths
+// 2 and 3; rows 5 and 6 fit the register file only at depth 2 (no spill
+// in the register-carry mode).  Depths 7 and 8 have room only for 8-wave
+// tiles of 3 or 4 rows (the levels' state takes ~40 registers per row).
 template <int KB>
-int launch_kb(int ry, int waves, float *uo, const float *ui, const float *kf,
-              const uint16_t *kb, uint16_t *co, const uint16_t *ci,
-              const h3d_grid *g, float factor, int mode, int zchunk,
-              hipStream_t s) {
-  if (ry == 2 && waves == 4) return launch_kb_ry_w<KB, 2, 4>(uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
-  if (ry == 2 && waves == 8) return launch_kb_ry_w<KB, 2, 8>(uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
-  if (ry == 4 && waves == 4) return launch_kb_ry_w<KB, 4, 4>(uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
-  if (ry == 4 && waves == 8) return launch_kb_ry_w<KB, 4, 8>(uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
+int launch_kb(int ry, int waves, const F32Launch &a, int mode) {
+  if constexpr (KB <= 6) {
+    if (ry == 2 && waves == 4) return launch_kb_ry_w<KB, 2, 4>(a, mode);
+    if (ry == 2 && waves == 8) return launch_kb_ry_w<KB, 2, 8>(a, mode);
+    if (ry == 4 && waves == 4) return launch_kb_ry_w<KB, 4, 4>(a, mode);
+  }
+  if (ry == 4 && waves == 8) return launch_kb_ry_w<KB, 4, 8>(a, mode);
+  if constexpr (KB <= 3) {
+    if (ry == 2 && waves == 16) return launch_kb_ry_w<KB, 2, 16>(a, mode);
+    if (ry == 3 && waves == 16) return launch_kb_ry_w<KB, 3, 16>(a, mode);
+    if (ry == 4 && waves == 16) return launch_kb_ry_w<KB, 4, 16>(a, mode);
+  }
+  if constexpr (KB == 2) {
+    if (ry == 5 && waves == 16) return launch_kb_ry_w<KB, 5, 16>(a, mode);
+    if (ry == 6 && waves == 16) return launch_kb_ry_w<KB, 6, 16>(a, mode);
+  }
+  if constexpr (KB >= 7) {
+    if (ry == 3 && waves == 8) return launch_kb_ry_w<KB, 3, 8>(a, mode);
+  }
   return H3D_EARG;
+}
+
+int launch_pass(int kb, int ry, int waves, const F32Launch &a, int mode) {
+  switch (kb) {
+#define H3D_KB(K) \
+    case K: return launch_kb<K>(ry, waves, a, mode);
+    H3D_KB(2) H3D_KB(3) H3D_KB(4) H3D_KB(5) H3D_KB(6) H3D_KB(7) H3D_KB(8)
+#undef H3D_KB
+  }
+  return H3D_ELIMIT;
 }
 
 // Single-step passes (KB = 1): wave-independent tiles of 64 x (waves * RY)
@@ -191,7 +239,11 @@ int launch_march(int ry, int waves, float *uo, const float *ui, const float *kf,
 
 }  // namespace
 
-int h3d_default_tune_f32(const h3d_grid *g, int kblock, h3d_tune *t) {
+// Launch shape of an FP32 pass of `kblock` steps when the caller names
+// none: single steps on the wave-independent tile; fused passes on the
+// 16-wave tiles where the depth has them (rows 5 at two steps, rows 4 at
+// three), else rows 4 x waves 8; the z chunk from h3d_round_zchunk.
+int h3d_default_tune_f32(const h3d_grid *g, int kblock, int mode, int bf16, h3d_tune *t) {
   t->kblock = kblock > 0 ? kblock : 2;
   if (t->kblock == 1) {
     // single-step tiles: the FP64 step's rule (whole z when that gives
@@ -206,17 +258,40 @@ int h3d_default_tune_f32(const h3d_grid *g, int kblock, h3d_tune *t) {
     t->zchunk = zc;
     return H3D_OK;
   }
-  t->rows = 2;
-  t->waves = 8;
-  const int OX = 64 - 2 * t->kblock, OY = t->rows * t->waves - 2 * t->kblock;
-  const long long tiles = (long long)((g->nx - 2 + OX - 1) / OX) *
-                          ((g->ny - 2 + (OY > 0 ? OY : 1) - 1) / (OY > 0 ? OY : 1));
-  const int planes = g->nz - 2;
-  int zc = planes;
-  // enough workgroups for 256 CUs x 2, z warm-up (2*kblock planes) small
-  while (zc > 8 * t->kblock && tiles * ((planes + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
+  t->rows = t->kblock == 2 ? 5 : t->kblock == 3 ? 4 : 4;
+  t->waves = t->kblock <= 3 ? 16 : 8;
+  int zc = 0, got = 0;
+  const int rc = h3d_pass_f32_zchunks(g, t->kblock, t->rows, t->waves, mode, bf16, &zc, 1, &got);
+  if (rc != H3D_OK) return rc;
+  if (got < 1) return H3D_EARG;
   t->zchunk = zc;
   return H3D_OK;
+}
+
+// Shape geometry: resident workgroups per CU and tiles per z chunk.
+int h3d_pass_f32_geometry(const h3d_grid *g, int kblock, int rows, int waves, int mode,
+                          int bf16, int *wg_per_cu, long long *tiles) {
+  F32Launch a{};
+  a.g = g;
+  a.bf16 = bf16 != 0;
+  a.occ = wg_per_cu;
+  a.tiles = tiles;
+  return launch_pass(kblock, rows, waves, a, mode);
+}
+
+// The n z chunks (into zc[], best first) that minimise the modelled pass
+// time of a shape: workgroups = tiles x chunks run in rounds of
+// CUs x (workgroups per CU), each workgroup marches its chunk plus 2*kblock
+// warm-up planes, so time ~ rounds x (chunk + 2*kblock).  A round left
+// part-empty costs as much as a full one, which is why the chunk count is
+// chosen to fill whole rounds.  *got receives how many were written.
+int h3d_pass_f32_zchunks(const h3d_grid *g, int kblock, int rows, int waves, int mode,
+                         int bf16, int *zc, int n, int *got) {
+  int occ = 0;
+  long long tiles = 0;
+  const int rc = h3d_pass_f32_geometry(g, kblock, rows, waves, mode, bf16, &occ, &tiles);
+  if (rc != H3D_OK) return rc;
+  return h3d_round_zchunks(g->nz - 2, kblock, tiles, occ, zc, n, got);
 }
 
 extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
@@ -228,26 +303,44 @@ extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
   if ((kappa_f32 == nullptr) == (kappa_bf16 == nullptr)) return H3D_EARG;
   if (mode < 0 || mode > 2) return H3D_EARG;
   if (mode == 2 && (!comp_out || !comp_in || comp_out == comp_in)) return H3D_EARG;
-  if (nsteps < 1 || nsteps > 6) return H3D_ELIMIT;
-  h3d_tune t;
-  h3d_default_tune_f32(g, nsteps, &t);
+  if (nsteps < 1 || nsteps > 8) return H3D_ELIMIT;
+  // the caller's non-zero tune fields, the rest from this depth's default
+  // shape; a z chunk left open is the modelled best for the shape in use
+  const int bf = kappa_bf16 != nullptr;
+  h3d_tune t{0, 0, 0, nsteps};
   if (tune) {
-    if (tune->rows) t.rows = tune->rows;
-    if (tune->waves) t.waves = tune->waves;
-    if (tune->zchunk) t.zchunk = tune->zchunk;
+    t.rows = tune->rows;
+    t.waves = tune->waves;
+    t.zchunk = tune->zchunk;
+  }
+  if (!t.rows || !t.waves) {
+    h3d_tune d;
+    const int dr = h3d_default_tune_f32(g, nsteps, mode, bf, &d);
+    if (dr != H3D_OK) return dr;
+    if (!t.zchunk && !t.rows && !t.waves) t.zchunk = d.zchunk;
+    if (!t.rows) t.rows = d.rows;
+    if (!t.waves) t.waves = d.waves;
+  }
+  if (!t.zchunk) {
+    if (nsteps == 1) {
+      h3d_tune d;
+      h3d_default_tune_f32(g, 1, mode, bf, &d);
+      t.zchunk = d.zchunk;
+    } else {
+      int got = 0;
+      const int r = h3d_pass_f32_zchunks(g, nsteps, t.rows, t.waves, mode, bf, &t.zchunk, 1, &got);
+      if (r != H3D_OK) return r;
+      if (got < 1) return H3D_EARG;
+    }
   }
   if (t.zchunk < 1) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
-  switch (nsteps) {
-    case 1:
-      return launch_march(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out,
-                          comp_in, g, factor, mode, t.zchunk, s);
-#define H3D_KB(K) \
-    case K: return launch_kb<K>(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor, mode, t.zchunk, s);
-    H3D_KB(2) H3D_KB(3) H3D_KB(4) H3D_KB(5) H3D_KB(6)
-#undef H3D_KB
-  }
-  return H3D_EARG;
+  if (nsteps == 1)
+    return launch_march(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out,
+                        comp_in, g, factor, mode, t.zchunk, s);
+  F32Launch a{u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor, t.zchunk,
+              s, bf != 0, nullptr, nullptr};
+  return launch_pass(nsteps, t.rows, t.waves, a, mode);
 }
 
 extern "C" int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
@@ -259,38 +352,50 @@ extern "C" int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;
   const int kb = (tune && tune->kblock > 0) ? tune->kblock : 2;
-  if (kb > 6) return H3D_ELIMIT;
+  if (kb > 8) return H3D_ELIMIT;
+  const int full = steps / kb, rem = steps % kb;
+  const int npass = full + (rem ? 1 : 0);
+  // pass p: ping-pong by parity; the last pass of a run whose step count
+  // is not a multiple of kb is shorter, on the caller's shape where that
+  // depth has it, else on that depth's own default shape
+  auto pass = [&](int p) {
+    const bool even = (p % 2) == 0;
+    const int ns = (p < full) ? kb : rem;
+    float *uo = even ? u_b : u_a;
+    const float *ui = even ? u_a : u_b;
+    uint16_t *co = even ? c_b : c_a;
+    const uint16_t *ci = even ? c_a : c_b;
+    int r = h3d_pass_f32(uo, ui, kappa_f32, kappa_bf16, co, ci, g, factor, ns, mode, tune, s);
+    if (r == H3D_EARG && p >= full)
+      r = h3d_pass_f32(uo, ui, kappa_f32, kappa_bf16, co, ci, g, factor, ns, mode, nullptr, s);
+    return r;
+  };
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
-  const int full = steps / kb, rem = steps % kb;
   int rc = H3D_OK;
-  if (graph && full >= 2) {
+  // graph: the run's passes captured once and launched as one graph (runs
+  // of more than H3D_GRAPH_PASSES passes replay a graph of that many
+  // full passes, the rest launched directly)
+  int in_graph = 0, reps = 0;
+  if (graph && npass >= 2) {
+    in_graph = npass <= H3D_GRAPH_PASSES ? npass : H3D_GRAPH_PASSES;
+    reps = npass <= H3D_GRAPH_PASSES ? 1 : full / H3D_GRAPH_PASSES;
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int r1 = h3d_pass_f32(u_b, u_a, kappa_f32, kappa_bf16, c_b, c_a, g, factor, kb, mode, tune, s);
-    int r2 = h3d_pass_f32(u_a, u_b, kappa_f32, kappa_bf16, c_a, c_b, g, factor, kb, mode, tune, s);
+    int r = H3D_OK;
+    for (int p = 0; r == H3D_OK && p < in_graph; ++p) r = pass(p);
     hipError_t ec = hipStreamEndCapture(s, &R.gr);
-    if (r1 != H3D_OK) return r1;
-    if (r2 != H3D_OK) return r2;
+    if (r != H3D_OK) return r;
     H3D_TRY(ec);
     H3D_TRY(hipGraphInstantiate(&R.ge, R.gr, nullptr, nullptr, 0));
   }
   H3D_TRY(hipEventRecord(R.e0, s));
-  int pass = 0;
-  if (R.ge) {
-    for (; pass + 2 <= full; pass += 2) {
-      hipError_t el = hipGraphLaunch(R.ge, s);
-      if (el != hipSuccess) { rc = (int)el; break; }
-    }
+  int p = 0;
+  for (int i = 0; i < reps && rc == H3D_OK; ++i, p += in_graph) {
+    hipError_t el = hipGraphLaunch(R.ge, s);
+    if (el != hipSuccess) rc = (int)el;
   }
-  const int npass = full + (rem ? 1 : 0);
-  for (; rc == H3D_OK && pass < npass; ++pass) {
-    const bool even = (pass % 2) == 0;
-    const int ns = (pass < full) ? kb : rem;
-    rc = h3d_pass_f32(even ? u_b : u_a, even ? u_a : u_b, kappa_f32, kappa_bf16,
-                      even ? c_b : c_a, even ? c_a : c_b, g, factor, ns, mode,
-                      tune, s);
-  }
+  for (; rc == H3D_OK && p < npass; ++p) rc = pass(p);
   if (rc == H3D_OK) {
     H3D_TRY(hipEventRecord(R.e1, s));
     H3D_TRY(hipEventSynchronize(R.e1));

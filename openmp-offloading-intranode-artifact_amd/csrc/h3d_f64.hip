@@ -316,33 +316,37 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
                    : h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, 1, g->nz - 2, tune, s);
   };
   const int full = steps / kb, rem = steps % kb;
+  const int npass = full + (rem ? 1 : 0);
+  auto pass_p = [&](int p) {
+    const bool even = (p % 2) == 0;
+    return pass(even ? u_b : u_a, even ? u_a : u_b, p < full ? kb : rem);
+  };
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
   int rc = H3D_OK;
-  if (graph && full >= 2) {
+  // graph: the run's passes captured once and launched as one graph (runs
+  // of more than H3D_GRAPH_PASSES passes replay a graph of that many full
+  // passes, the rest launched directly)
+  int in_graph = 0, reps = 0;
+  if (graph && npass >= 2) {
+    in_graph = npass <= H3D_GRAPH_PASSES ? npass : H3D_GRAPH_PASSES;
+    reps = npass <= H3D_GRAPH_PASSES ? 1 : full / H3D_GRAPH_PASSES;
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    int r1 = pass(u_b, u_a, kb);
-    int r2 = pass(u_a, u_b, kb);
+    int r = H3D_OK;
+    for (int p = 0; r == H3D_OK && p < in_graph; ++p) r = pass_p(p);
     hipError_t ec = hipStreamEndCapture(s, &R.gr);
-    if (r1 != H3D_OK) return r1;
-    if (r2 != H3D_OK) return r2;
+    if (r != H3D_OK) return r;
     H3D_TRY(ec);
     H3D_TRY(hipGraphInstantiate(&R.ge, R.gr, nullptr, nullptr, 0));
   }
   H3D_TRY(hipEventRecord(R.e0, s));
   int done = 0;
-  if (R.ge) {
-    for (; done + 2 <= full; done += 2) {
-      hipError_t el = hipGraphLaunch(R.ge, s);
-      if (el != hipSuccess) { rc = (int)el; break; }
-    }
+  for (int i = 0; i < reps && rc == H3D_OK; ++i, done += in_graph) {
+    hipError_t el = hipGraphLaunch(R.ge, s);
+    if (el != hipSuccess) rc = (int)el;
   }
-  const int npass = full + (rem ? 1 : 0);
-  for (; rc == H3D_OK && done < npass; ++done) {
-    const bool even = (done % 2) == 0;
-    rc = pass(even ? u_b : u_a, even ? u_a : u_b, done < full ? kb : rem);
-  }
+  for (; rc == H3D_OK && done < npass; ++done) rc = pass_p(done);
   if (rc == H3D_OK) {
     H3D_TRY(hipEventRecord(R.e1, s));
     H3D_TRY(hipEventSynchronize(R.e1));

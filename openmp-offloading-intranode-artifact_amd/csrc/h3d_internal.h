@@ -111,3 +111,22 @@ __device__ __forceinline__ float h3d_from_right(float v, float edge) {
 // Library-internal entry points shared between translation units.
 int h3d_default_tune_f64(const h3d_grid *g, int k_start, int k_end,
                          h3d_tune *t);
+
+// Passes captured in one hipGraph by the run functions (even, so that a
+// replay keeps the ping-pong parity); longer runs replay it.
+#define H3D_GRAPH_PASSES 1024
+
+// FP32 pass shapes: default launch shape of a depth, a shape's geometry
+// (resident workgroups per CU, tiles per z chunk) and its best z chunks.
+int h3d_default_tune_f32(const h3d_grid *g, int kblock, int mode, int bf16, h3d_tune *t);
+int h3d_pass_f32_geometry(const h3d_grid *g, int kblock, int rows, int waves, int mode,
+                          int bf16, int *wg_per_cu, long long *tiles);
+int h3d_pass_f32_zchunks(const h3d_grid *g, int kblock, int rows, int waves, int mode,
+                         int bf16, int *zc, int n, int *got);
+
+// Up to n z-chunk lengths (best first) for a pass of `kblock` steps over
+// `planes` planes whose launch has `tiles` workgroups per z chunk and
+// `wg_per_cu` resident workgroups per CU; *got receives how many were
+// written.
+int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, int *zc, int n,
+                      int *got);

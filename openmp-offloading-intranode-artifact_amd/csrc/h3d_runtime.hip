@@ -64,9 +64,16 @@ extern "C" int h3d_target_memcpy(void *dst, const void *src, size_t bytes,
   if (rc != H3D_OK) return rc;
   if (kind == hipMemcpyDeviceToDevice && dst_device != src_device) {
     H3D_TRY(hipMemcpyPeer(dst, dst_device, src, src_device, bytes));
+    H3D_TRY(hipDeviceSynchronize());
     return H3D_OK;
   }
   H3D_TRY(hipMemcpy(dst, src, bytes, kind));
+  // hipMemcpy may return before the copy has landed (device-to-device
+  // copies, and the DMA leg of a staged copy from pageable memory); wait
+  // for it, so that work the caller queues next on any stream (including
+  // non-blocking ones, which do not order behind the null stream) sees the
+  // copied data
+  H3D_TRY(hipStreamSynchronize(nullptr));
   return H3D_OK;
 }
 
@@ -291,5 +298,123 @@ extern "C" int h3d_transfer_probe(size_t bytes, int reps, int pinned, int device
   out[1] = bytes / s[1] / 1e9;
   out[2] = s[2] * 1e6;
   out[3] = s[3] * 1e6;
+  return H3D_OK;
+}
+
+// One host<->device transfer measured as the reference's transfer
+// benchmarks do (Bandwidth_Measurements/AMD-MI250X/Synchronous-test-H-to-D-
+// Measurements/hip-1.hip:67-92 and its D-to-H twin; Asynchronous-test-H-to-
+// D-Measurements/hip-1.hip:79-98): two untimed warm-up copies, then one
+// timed copy -- the host clock around a blocking hipMemcpy (async = 0), or
+// HIP events on a stream around hipMemcpyAsync (async = 1) -- then a check
+// of the data that arrived.  The reference fills its buffer with 1.0 and
+// spot-checks 1000 of the first 1000 elements; here every 64-bit word i
+// carries its own value (a multiplicative hash of i), so a block landing
+// at a wrong offset fails too, and the check covers every word up to
+// 1 GiB and above that the first and last 4096 words plus 4096 spread
+// through the buffer.  direction 0 = H2D (the device copy is read back to
+// check it), 1 = D2H (the device buffer is filled by an H2D copy first).
+// out[0] GB/s (1e9 B/s), out[1] seconds of the timed copy, out[2] words
+// that did not arrive intact (0 when verified), out[3] words checked.
+#include <vector>
+
+static inline uint64_t xfer_word(size_t i) {
+  return (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x243F6A8885A308D3ull;
+}
+
+extern "C" int h3d_transfer_bench(size_t bytes, int pinned, int async, int direction,
+                                  int device_id, double *out) {
+  if (bytes < 8 || bytes % 8 || !out || (direction != 0 && direction != 1) ||
+      (async != 0 && async != 1))
+    return H3D_EARG;
+  int rc = set_dev(device_id);
+  if (rc != H3D_OK) return rc;
+  const size_t words = bytes / 8;
+  uint64_t *h = nullptr, *d = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = pinned ? hipHostMalloc((void **)&h, bytes, hipHostMallocDefault)
+                        : (h = (uint64_t *)malloc(bytes)) ? hipSuccess : hipErrorOutOfMemory;
+  if (e == hipSuccess) e = hipMalloc((void **)&d, bytes);
+  if (e == hipSuccess) e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  if (e == hipSuccess) {
+    for (size_t i = 0; i < words; ++i) h[i] = xfer_word(i);
+    if (direction == 1) {  // device holds the pattern, host is cleared
+      e = hipMemcpy(d, h, bytes, hipMemcpyHostToDevice);
+      if (e == hipSuccess) memset(h, 0, bytes);
+    }
+  }
+  void *dst = direction == 0 ? (void *)d : (void *)h;
+  const void *src = direction == 0 ? (const void *)h : (const void *)d;
+  const hipMemcpyKind kind = direction == 0 ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  auto copy = [&]() {
+    if (!async) return hipMemcpy(dst, src, bytes, kind);
+    hipError_t r = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return r == hipSuccess ? hipStreamSynchronize(s) : r;
+  };
+  for (int w = 0; w < 2 && e == hipSuccess; ++w) e = copy();
+  double sec = 0.0;
+  if (e == hipSuccess && !async) {
+    const auto t0 = std::chrono::steady_clock::now();
+    e = hipMemcpy(dst, src, bytes, kind);
+    sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  } else if (e == hipSuccess) {
+    float ms = 0.f;
+    e = hipEventRecord(e0, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    sec = ms * 1e-3;
+  }
+  // the check: which words to look at, and where their values are
+  std::vector<size_t> idx;
+  const bool all = bytes <= (1ull << 30);
+  if (e == hipSuccess && !all) {
+    const size_t edge = words < 4096 ? words : 4096;
+    for (size_t i = 0; i < edge; ++i) idx.push_back(i);
+    for (size_t i = words - edge; i < words; ++i) idx.push_back(i);
+    for (size_t i = 0; i < 4096; ++i) idx.push_back((size_t)(xfer_word(i) % words));
+  }
+  double bad = 0, checked = 0;
+  if (e == hipSuccess && direction == 1) {  // the host buffer received the data
+    if (all) {
+      for (size_t i = 0; i < words; ++i) bad += h[i] != xfer_word(i);
+      checked = (double)words;
+    } else {
+      for (size_t i : idx) bad += h[i] != xfer_word(i);
+      checked = (double)idx.size();
+    }
+  } else if (e == hipSuccess) {  // read the device copy back
+    if (all) {
+      std::vector<uint64_t> back(words);
+      e = hipMemcpy(back.data(), d, bytes, hipMemcpyDeviceToHost);
+      for (size_t i = 0; e == hipSuccess && i < words; ++i) bad += back[i] != xfer_word(i);
+      checked = (double)words;
+    } else {
+      for (size_t i : idx) {
+        uint64_t v = 0;
+        e = hipMemcpy(&v, d + i, 8, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) break;
+        bad += v != xfer_word(i);
+      }
+      checked = (double)idx.size();
+    }
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (d) (void)hipFree(d);
+  if (h) {
+    if (pinned) (void)hipHostFree(h);
+    else free(h);
+  }
+  if (e != hipSuccess) return (int)e;
+  out[0] = sec > 0 ? bytes / sec / 1e9 : 0.0;
+  out[1] = sec;
+  out[2] = bad;
+  out[3] = checked;
   return H3D_OK;
 }

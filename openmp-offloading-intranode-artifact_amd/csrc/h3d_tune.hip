@@ -59,6 +59,40 @@ int time_best(hipStream_t s, F launch, int nl, float *best_ms) {
 
 }  // namespace
 
+int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, int *zc, int n,
+                      int *got) {
+  if (planes < 1 || tiles < 1 || !zc || n < 1 || !got) return H3D_EARG;
+  *got = 0;
+  int dev = 0, cus = 0;
+  H3D_TRY(hipGetDevice(&dev));
+  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const long long slots = (long long)(wg_per_cu > 0 ? wg_per_cu : 1) * (cus > 0 ? cus : 1);
+  struct C {
+    long long cost, rounds;
+    int z;
+  };
+  std::vector<C> v;
+  int last = -1;
+  for (int c = 1; c <= planes; ++c) {
+    const int z = (planes + c - 1) / c;
+    if (z == last) continue;
+    last = z;
+    if (c > 1 && z < 4 * kblock) break;  // warm-up would exceed half the work
+    const long long wgs = tiles * ((planes + z - 1) / z);
+    const long long rounds = (wgs + slots - 1) / slots;
+    v.push_back({rounds * (long long)(z + 2 * kblock), rounds, z});
+  }
+  // cheapest first; equal cost: fewer rounds (neighbouring tiles march
+  // together in one round, so their shared halo lines meet in L2)
+  std::stable_sort(v.begin(), v.end(), [](const C &a, const C &b) {
+    return a.cost != b.cost ? a.cost < b.cost : a.rounds < b.rounds;
+  });
+  int k = 0;
+  for (size_t i = 0; i < v.size() && k < n; ++i) zc[k++] = v[i].z;
+  *got = k;
+  return H3D_OK;
+}
+
 extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
                             int kappa_bf16, const h3d_grid *g, int mode,
                             int fixed_kblock, void *stream, h3d_tune *out) {
@@ -67,7 +101,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   if (kind != 0 && kind != 1) return H3D_EARG;
   if (kind == 1 && (mode < 0 || mode > 2)) return H3D_EARG;
   if (kind == 0 && (mode < 0 || mode > 1)) return H3D_EARG;
-  if (fixed_kblock < 0 || fixed_kblock > (kind ? 6 : 4)) return H3D_ELIMIT;
+  if (fixed_kblock < 0 || fixed_kblock > (kind ? 8 : 4)) return H3D_ELIMIT;
   const Key key{kind, g->nx, g->ny, g->nz, g->sy, mode,
                 kind ? kappa_bf16 : 0, fixed_kblock, g->sz};
   {
@@ -119,7 +153,8 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   std::vector<std::pair<float, h3d_tune>> cand;  // (ms per step, shape)
   if (rc == H3D_OK && kind == 0) {
     const int planes = g->nz - 2;
-    for (int kbk = 1; kbk <= 3 && rc == H3D_OK; ++kbk) {
+    // depths 1..3 when the tuner chooses; a fixed depth (up to 4) alone
+    for (int kbk = 1; kbk <= (fixed_kblock ? fixed_kblock : 3) && rc == H3D_OK; ++kbk) {
       if (fixed_kblock && kbk != fixed_kblock) continue;
       // single steps: whole z range first (on large grids the chunk-edge
       // re-reads cost more than the extra parallelism of short chunks
@@ -142,24 +177,46 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
     }
   } else if (rc == H3D_OK) {
     const int planes = g->nz - 2;
-    for (int kbk = 1; kbk <= 4 && rc == H3D_OK; ++kbk) {
+    // depths 1..4 when the tuner chooses; a fixed depth (up to 8) alone
+    for (int kbk = 1; kbk <= (fixed_kblock ? fixed_kblock : 4) && rc == H3D_OK; ++kbk) {
       if (fixed_kblock && kbk != fixed_kblock) continue;
       // single-step passes use the wave-independent tiles (rows 1-4, any
-      // wave count, long z marches); fused passes the LDS-shared tiles
-      const std::vector<int> rws = kbk == 1 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4};
-      const std::vector<int> wvs = kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8};
-      const std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
-                                            : std::vector<int>{32, 48, 64, 96, 128};
-      for (int rows : rws)
-        for (int waves : wvs)
-          for (int zc : zcs) {
-            if (zc > planes && zc != zcs.front()) continue;
-            h3d_tune t{rows, waves, zc, kbk};
-            float ms;
-            int r = time_best(s, launcher(t), 2, &ms);
-            if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
-            if (r == H3D_OK) cand.push_back({ms / kbk, t});
-          }
+      // wave count, long z marches); fused passes the LDS-shared tiles in
+      // every shape the depth offers, each with the z chunks that fill
+      // whole rounds of resident workgroups (h3d_round_zchunks) and 64
+      std::vector<std::pair<int, int>> shapes;
+      if (kbk == 1) {
+        for (int rows : {1, 2, 4})
+          for (int waves : {2, 4, 8}) shapes.push_back({rows, waves});
+      } else {
+        shapes = {{2, 4}, {2, 8}, {4, 4}, {4, 8}, {2, 16}, {3, 16},
+                  {4, 16}, {5, 16}, {6, 16}, {3, 8}};
+      }
+      for (const auto &sh : shapes) {
+        std::vector<int> zcs;
+        if (kbk == 1) {
+          zcs = {planes, 256, 128, 64, 32};
+        } else {
+          int occ = 0;
+          long long tl = 0;
+          if (h3d_pass_f32_geometry(g, kbk, sh.first, sh.second, mode, kappa_bf16, &occ, &tl) !=
+              H3D_OK)
+            continue;  // shape not offered at this depth
+          int z[3], nz3 = 0;
+          if (h3d_round_zchunks(planes, kbk, tl, occ, z, 3, &nz3) != H3D_OK) nz3 = 0;
+          for (int i = 0; i < nz3; ++i) zcs.push_back(z[i]);
+          if (std::find(zcs.begin(), zcs.end(), 64) == zcs.end() && planes > 64) zcs.push_back(64);
+        }
+        for (int zc : zcs) {
+          if (zc > planes && zc != zcs.front()) continue;
+          h3d_tune t{sh.first, sh.second, zc, kbk};
+          float ms;
+          int r = time_best(s, launcher(t), 2, &ms);
+          if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
+          if (r == H3D_OK) cand.push_back({ms / kbk, t});
+        }
+        if (rc != H3D_OK) break;
+      }
     }
   }
   // the four fastest again, over ten launches per sample

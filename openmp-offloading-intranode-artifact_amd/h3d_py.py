@@ -97,6 +97,7 @@ SIGNATURES = {
     "h3d_device_arch": (_I, [_I, C.c_char_p, _I]),
     "h3d_stream_probe": (_I, [C.c_size_t, _I, _I, C.POINTER(_D), C.POINTER(_D)]),
     "h3d_transfer_probe": (_I, [C.c_size_t, _I, _I, _I, _P]),
+    "h3d_transfer_bench": (_I, [C.c_size_t, _I, _I, _I, _I, _P]),
     "h3d_params_reference": (_I, [_I, C.POINTER(_D), C.POINTER(_D), C.POINTER(_D)]),
     "h3d_count_steps": (_I, [_D, _D]),
     "h3d_init_reference": (_I, [_P, _P, _P, _GP]),
@@ -322,6 +323,14 @@ def stream_probe(nbytes, reps=5, device=0):
     c, t = C.c_double(), C.c_double()
     call("h3d_stream_probe", C.c_size_t(nbytes), reps, device, C.byref(c), C.byref(t))
     return c.value, t.value
+
+
+def transfer_bench(nbytes, pinned=True, asynchronous=False, d2h=False, device=0):
+    """(GB/s, seconds, words wrong, words checked) of one verified copy."""
+    out = np.zeros(4)
+    call("h3d_transfer_bench", C.c_size_t(nbytes), int(pinned), int(asynchronous),
+         int(d2h), device, ptr(out))
+    return float(out[0]), float(out[1]), int(out[2]), int(out[3])
 
 
 def transfer_probe(nbytes, reps=3, pinned=True, device=0):

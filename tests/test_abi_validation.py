@@ -112,12 +112,12 @@ def test_fp32_pass_checks(h3d, L):
     assert P(u, v, kf, None, None, None, gp, f, 1, 2, None, None) == EARG    # mode 2: residuals
     assert P(u, v, kf, None, c1, c1, gp, f, 1, 2, None, None) == EARG        # in place
     assert P(u, v, kf, None, None, None, gp, f, 0, 0, None, None) == ELIMIT
-    assert P(u, v, kf, None, None, None, gp, f, 7, 0, None, None) == ELIMIT
+    assert P(u, v, kf, None, None, None, gp, f, 9, 0, None, None) == ELIMIT
     for kblock, rows, waves in ((2, 1, 4), (2, 2, 2), (3, 8, 8), (1, 3, 4), (1, 2, 9)):
         t = h3d.Tune(rows, waves, 8, kblock)
         assert P(u, v, kf, None, None, None, gp, f, kblock, 0, C.byref(t), None) == EARG
     ms, fb = C.c_float(), C.c_int()
-    t = h3d.Tune(0, 0, 0, 7)
+    t = h3d.Tune(0, 0, 0, 9)
     assert L.h3d_run_f32(u, v, kf, None, None, None, gp, f, 10, 0, C.byref(t), 0, None,
                          C.byref(ms), C.byref(fb)) == ELIMIT
 
@@ -167,7 +167,7 @@ def test_ensemble_and_tuner_checks(h3d, L):
     u, v, k = FAKE[:3]
     assert L.h3d_autotune(2, u, v, k, 0, C.byref(g), 0, 0, None, C.byref(t)) == EARG
     assert L.h3d_autotune(1, u, v, k, 0, C.byref(g), 3, 0, None, C.byref(t)) == EARG
-    assert L.h3d_autotune(1, u, v, k, 0, C.byref(g), 1, 7, None, C.byref(t)) == ELIMIT
+    assert L.h3d_autotune(1, u, v, k, 0, C.byref(g), 1, 9, None, C.byref(t)) == ELIMIT
     assert L.h3d_autotune(0, u, v, k, 0, C.byref(g), 2, 0, None, C.byref(t)) == EARG  # FP64: rhs flag
     assert L.h3d_autotune(0, u, v, k, 0, C.byref(g), 0, 5, None, C.byref(t)) == ELIMIT
     bad = grid(h3d, nx=2)

@@ -234,3 +234,49 @@ def test_cfg5_64_members_500_steps_accuracy(gpu):
         assert np.all(np.isfinite(da.download()))
     gpu.call("h3d_ensemble_destroy", e)
     gpu.call("h3d_ensemble_destroy", e2)
+
+
+def test_cfg5_members_bitwise_first_steps(gpu, oracle):
+    """The 64-member configuration at full size with every member fused 3
+    steps per pass (the depth the benchmark's tuner picks): after 7 steps
+    (two 3-step passes and a single step) every FP64 member equals the
+    oracle's run bit for bit, and a sample of FP32 members equals the
+    oracle replaying the same passes with the FP16 residual carried."""
+    dims = cfg5_dims()
+    specs = [make_member(gpu, d, m, 20260101, m % 2) for m, d in enumerate(dims)]
+    mems, keep = [], []
+    for sp in specs:
+        da, db, dk = (gpu.DeviceArray.from_host(x) for x in (sp["u"], sp["u"], sp["k"]))
+        keep.append((da, db, dk))
+        mems.append(gpu.Member(da.ptr, db.ptr, dk.ptr, sp["g"], sp["factor"], sp["precision"]))
+    arr = (gpu.Member * len(mems))(*mems)
+    e = C.c_void_p()
+    t = gpu.Tune(0, 0, 0, 3)
+    gpu.call("h3d_ensemble_create", C.byref(e), arr, len(mems), C.byref(t), 0)
+    s = gpu.Stream()
+    steps = 7
+    run(gpu, e, steps, 1, s)
+    depths = pass_depths(steps, 3)
+    checked32 = 0
+    for m, (sp, (da, db, dk)) in enumerate(zip(specs, keep)):
+        g = sp["g"]
+        if sp["precision"] == 1 and m % 8 != 1:
+            continue  # FP32 sample: members 1, 9, 17, ... (the oracle's FP32 path is serial)
+        got = (db if where(gpu, e, m) else da).download()
+        if sp["precision"] == 0:
+            want = oracle.run_f64(sp["u"].copy(), sp["u"].copy(), None, sp["k"], g.nx, g.ny,
+                                  g.nz, steps, factor=sp["factor"], nthreads=16)
+            assert np.array_equal(got, want), (m, dims[m])
+        else:
+            f32 = float(np.float32(sp["factor"]))
+            u, comp = sp["u"], None
+            for d in depths:
+                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
+                                         comp0=comp)
+            assert got.tobytes() == u.tobytes(), (m, dims[m])
+            checked32 += 1
+    assert checked32 == 8
+    gpu.call("h3d_ensemble_destroy", e)
+    for trip in keep:
+        for d in trip:
+            d.close()

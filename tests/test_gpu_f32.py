@@ -69,6 +69,20 @@ CASES = [
     ((9, 75, 12), 1, 4, 4, 11),
     ((70, 40, 30), 1, 1, 2, 0),
     ((130, 35, 19), 1, 4, 8, 7),
+    # 16-wave tiles (one 1024-thread workgroup per CU) of 2..6 rows per lane
+    ((70, 40, 30), 2, 2, 16, 0),
+    ((70, 100, 30), 2, 4, 16, 13),
+    ((130, 90, 21), 2, 5, 16, 0),
+    ((70, 110, 17), 2, 6, 16, 9),
+    ((130, 70, 21), 3, 4, 16, 0),
+    ((65, 50, 21), 3, 3, 16, 7),
+    # deep passes: 7 and 8 steps on 8-wave tiles of 3 and 4 rows (the
+    # remainder pass of a run on the depth's default shape where the
+    # single-step tile has no 3-row form)
+    ((130, 35, 40), 8, 3, 8, 0),
+    ((70, 60, 30), 8, 4, 8, 11),
+    ((65, 40, 25), 7, 3, 8, 0),
+    ((65, 40, 25), 5, 4, 8, 0),
 ]
 
 

@@ -360,3 +360,56 @@ def test_transfer_probe_runs(gpu, pinned):
     L = gpu.lib()
     out = np.zeros(4)
     assert L.h3d_transfer_probe(100, 1, 1, 0, gpu.ptr(out)) == -1
+
+
+@pytest.mark.parametrize("nbytes", [8, 4096, (1 << 20) + 8, 1 << 26], ids=str)
+@pytest.mark.parametrize("pinned", [True, False])
+@pytest.mark.parametrize("asynchronous", [False, True])
+@pytest.mark.parametrize("d2h", [False, True])
+def test_transfer_bench_delivers_every_word(gpu, nbytes, pinned, asynchronous, d2h):
+    """The verified transfer measurement (the reference's synchronous and
+    asynchronous H2D / D2H benchmarks): every 64-bit word arrives with its
+    own value, in both directions, from pinned and pageable memory, and
+    the rate is a positive finite number."""
+    gbs, sec, bad, checked = gpu.transfer_bench(nbytes, pinned, asynchronous, d2h)
+    assert bad == 0 and checked == nbytes // 8
+    assert sec > 0 and math.isfinite(gbs) and gbs > 0
+
+
+def test_transfer_bench_argument_errors(gpu):
+    L = gpu.lib()
+    out = np.zeros(4)
+    assert L.h3d_transfer_bench(12, 1, 0, 0, 0, gpu.ptr(out)) == -1   # not whole words
+    assert L.h3d_transfer_bench(0, 1, 0, 0, 0, gpu.ptr(out)) == -1
+    assert L.h3d_transfer_bench(64, 1, 2, 0, 0, gpu.ptr(out)) == -1   # async flag
+    assert L.h3d_transfer_bench(64, 1, 0, 2, 0, gpu.ptr(out)) == -1   # direction
+    assert L.h3d_transfer_bench(64, 1, 0, 0, 0, None) == -1
+    assert L.h3d_transfer_bench(64, 1, 0, 0, 99, gpu.ptr(out)) == -3  # no such device
+
+
+def test_device_copy_completes_before_return(gpu):
+    """h3d_target_memcpy is synchronous for device-to-device copies too:
+    a pass queued right after it on a non-blocking stream reads the copied
+    field, so the result equals the same pass on a field uploaded from the
+    host (the copy of 1 GiB takes longer than the launch of the pass)."""
+    n = 514
+    g = gpu.Grid.dense(n)
+    u, k = gpu.gen_seeded(g, 0, n, n, n, 0, 11)
+    u32, k32 = u.astype(np.float32), k.astype(np.float32)
+    src = gpu.DeviceArray.from_host(u32)
+    dk = gpu.DeviceArray.from_host(k32)
+    s = gpu.Stream()
+    outs = []
+    for via_copy in (False, True):
+        a = gpu.DeviceArray.from_host(np.zeros_like(u32))
+        b = gpu.DeviceArray.from_host(u32)
+        if via_copy:
+            gpu.call("h3d_target_memcpy", a.ptr, src.ptr, gpu.C.c_size_t(src.nbytes), 0, 0)
+        else:
+            a.upload(u32)
+        t = gpu.Tune(0, 0, 0, 2)
+        gpu.call("h3d_pass_f32", b.ptr, a.ptr, dk.ptr, None, None, None, gpu.C.byref(g),
+                 gpu.C.c_float(0.01), 2, 1, gpu.C.byref(t), s.ptr)
+        s.sync()
+        outs.append(b.download())
+    assert outs[0].tobytes() == outs[1].tobytes()

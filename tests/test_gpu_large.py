@@ -99,24 +99,78 @@ def test_1000_steps_invariants(gpu, big):
 def test_fp32_compensated_target_accuracy(gpu, big):
     """The accuracy target of the FP32 path: FP32 storage, compensated,
     temporally blocked, 1000 steps at 512^3: rel-L2 <= 1e-5 against the FP64
-    kernel on the same (FP32-rounded) inputs."""
+    kernel on the same (FP32-rounded, or BF16-dequantised) inputs, at every
+    fused depth the benchmark runs (2 and 3 on their default 16-wave tiles,
+    4 and 8) and with BF16 kappa.  Plain FP32 (mode 0) has an absolute bound
+    too, and the register carry must beat it."""
     g = big["g"]
     u32 = big["u"].astype(np.float32)
     k32 = big["k"].astype(np.float32)
-    ref, _ = run64(gpu, big, u32.astype(np.float64), k32.astype(np.float64), 1000)
+    bits, scale = gpu.quantize_bf16(big["k"])
+    refs = {}
+    for name, k64 in (("f32", k32.astype(np.float64)), ("bf16", gpu.bf16_to_f64(bits, scale))):
+        r, other = run64(gpu, big, u32.astype(np.float64), k64, 1000)
+        other.close()
+        refs[name] = r
     da, db = gpu.DeviceArray.from_host(u32), gpu.DeviceArray.from_host(u32)
     dk = gpu.DeviceArray.from_host(k32)
-    f32f = gpu.C.c_float(np.float32(big["factor"]))
+    dkb = gpu.DeviceArray.from_host(bits)
     results = {}
-    for kb in (2, 4):
+    for kname, kb, mode in (("f32", 2, 1), ("f32", 3, 1), ("f32", 4, 1), ("f32", 8, 1),
+                            ("f32", 2, 0), ("bf16", 2, 1), ("bf16", 3, 1)):
         da.upload(u32)
         db.upload(u32)
         ms, fb = gpu.C.c_float(), gpu.C.c_int()
         t = gpu.Tune(0, 0, 0, kb)
-        gpu.call("h3d_run_f32", da.ptr, db.ptr, dk.ptr, None, None, None, gpu.C.byref(g),
-                 f32f, 1000, 1, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
-        m = gpu.compare(ref, db if fb.value else da, g, f32=True)
-        results[kb] = (m[3], m[0])
+        kf, kbf = (dk.ptr, None) if kname == "f32" else (None, dkb.ptr)
+        f32f = gpu.C.c_float(np.float32(big["factor"] * (1.0 if kname == "f32" else scale)))
+        gpu.call("h3d_run_f32", da.ptr, db.ptr, kf, kbf, None, None, gpu.C.byref(g),
+                 f32f, 1000, mode, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+        m = gpu.compare(refs[kname], db if fb.value else da, g, f32=True)
+        results[(kname, kb, mode)] = (m[3], m[0])
     print("rel-L2, max-abs vs FP64:", results)
-    for kb, (rel_l2, max_abs) in results.items():
-        assert rel_l2 <= 1e-5, kb
+    for key, (rel_l2, max_abs) in results.items():
+        assert rel_l2 <= (2e-5 if key[2] == 0 else 1e-5), key
+    assert results[("f32", 2, 1)][0] < results[("f32", 2, 0)][0]
+    for r in refs.values():
+        r.close()
+
+
+def test_slab_configuration_full_shape_vs_oracle(gpu, oracle):
+    """The slab configuration at its real shape: one GPU's slab of a 4-way
+    z split of 1024^3 (1026 x 1026 x 258 points, seeded lognormal kappa, the
+    aligned pitched layout), 10 steps of h3d_run_slab_f64 (boundary planes,
+    interior on a second stream, faces packed and unpacked in HBM into the
+    slab's own ghost planes, replayed from a graph) against the oracle
+    stepping the whole slab and copying the same planes
+    (4-openmp-stream-omp-p2p.c:609-616), bit for bit."""
+    n, planes, steps = 1026, 258, 10
+    rc, nzd, gks = gpu.slab_decompose(n, 4)
+    assert rc == 0 and nzd[1] == planes
+    g = gpu.pitched_grid(n, n, planes)
+    u, k = gpu.gen_seeded(g, gks[1] - 1, n, n, n, 1, 20260101)
+    h = 1.0 / (n - 1)
+    fmax = gpu.lib().h3d_face_sum_max_host(gpu.ptr(k), gpu.C.byref(g))
+    factor = 0.9 * 2.0 * h * h / fmax / h / h / 2.0
+    da, db, dk = (gpu.AlignedField(g, np.float64) for _ in range(3))
+    da.upload(u)
+    db.upload(u)
+    dk.upload(k)
+    lo, hi = gpu.DeviceArray(n * n * 8), gpu.DeviceArray(n * n * 8)
+    s = gpu.Stream()
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    gpu.call("h3d_run_slab_f64", da.ptr, db.ptr, None, dk.ptr, gpu.C.byref(g), factor,
+             lo.ptr, hi.ptr, steps, None, 1, s.ptr, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = (db if fb.value else da).download()
+    for f in (da, db, dk, lo, hi):
+        f.close()
+    a, b = u.copy(), u.copy()
+    P = g.sz
+    for _ in range(steps):
+        cur = oracle.run_f64(a, b, None, k, n, n, planes, 1, sy=g.sy, sz=g.sz,
+                             factor=factor, nthreads=16)
+        nxt = a if cur is b else b
+        cur[0:P] = cur[(planes - 2) * P:(planes - 1) * P]
+        cur[(planes - 1) * P:planes * P] = cur[P:2 * P]
+        a, b = cur, nxt
+    assert np.array_equal(got, a)
