@@ -205,6 +205,20 @@ int h3d_stats_f64(const double *u, const h3d_grid *g, double *out,
 int h3d_stats_f32(const float *u, const h3d_grid *g, double *out,
                   void *stream);
 
+/* h3d_stats_f64 for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The partial records live in a fixed
+ * device array of the library (1024 records of 48 bytes, one copy per GPU,
+ * held until the process ends) instead of a stream-ordered allocation, and
+ * the four results (same meaning and bits as h3d_stats_f64's out[]) are
+ * written to the caller's DEVICE memory out_dev, ordered on `stream`,
+ * without synchronising.  Calls that overlap on one GPU share the array
+ * and race: order them on one stream.  H3D_ELIMIT if the field would need
+ * more partial records than the array holds (h3d_stats_f64 uses at most
+ * 1024 for any grid, so every valid grid fits). */
+int h3d_stats_f64_noalloc(const double *u, const h3d_grid *g, double *out_dev,
+                          void *stream);
+
 /* Error metrics of compare.py:47-76 and the PASS rule of verify.sh:239-286,
  * computed on the device in FP64 over the whole array:
  * out[0] max_abs, [1] mean_abs, [2] rms, [3] rel_l2, [4] max_rel over

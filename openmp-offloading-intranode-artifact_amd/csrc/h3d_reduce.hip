@@ -10,6 +10,8 @@
 // are HBM-bound like the stencil.
 #include <math.h>
 
+#include <mutex>
+
 #include "h3d_internal.h"
 
 namespace {
@@ -299,6 +301,76 @@ extern "C" int h3d_compare_f32(const double *ref, const float *cand,
                                double *out, void *stream) {
   return cmp_impl<float>(ref, cand, g, atol, rtol, out, stream);
 }
+// ---- h3d_stats_f64 without allocation ----------------------------------
+//
+// The same two passes as h3d_stats_f64, with the partial records in a
+// fixed array of the library's own device memory (one copy per GPU, kept
+// until the process ends) instead of a stream-ordered allocation, and the
+// four results written to the caller's device memory, ordered on the
+// stream, with no host synchronisation.  For callers that cannot allocate
+// device memory (code under graph capture, loops that forbid hipMalloc).
+// The partition into records is h3d_stats_f64's, so the results carry the
+// same bits.  Every record a call folds is written by that call first, so
+// nothing needs clearing between calls; calls that overlap on one GPU
+// share the array and race.
+
+namespace {
+
+constexpr int NOALLOC_PARTS = MAXPART;  // records: 1024 x 48 bytes
+__device__ __align__(256) Stats noalloc_scratch[NOALLOC_PARTS];
+
+// Device address of this device's copy of noalloc_scratch, looked up once
+// per device with that device current (the array's name is a host-side
+// handle, not an address a kernel may use).
+std::mutex noalloc_mu;
+Stats *noalloc_ptr[64] = {};
+
+int noalloc_part(Stats **out) {
+  int dev = 0;
+  H3D_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return H3D_ELIMIT;
+  std::lock_guard<std::mutex> lk(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void *p = nullptr;
+    H3D_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch)));
+    noalloc_ptr[dev] = static_cast<Stats *>(p);
+  }
+  *out = noalloc_ptr[dev];
+  return H3D_OK;
+}
+
+__global__ void stats_emit(const Stats *part, double *out) {
+  if (threadIdx.x == 0) {
+    const Stats r = part[0];
+    out[0] = r.sum.s + r.sum.c;
+    out[1] = r.sq.s + r.sq.c;
+    out[2] = r.mx;
+    out[3] = r.bad;
+  }
+}
+
+}  // namespace
+
+extern "C" int h3d_stats_f64_noalloc(const double *u, const h3d_grid *g, double *out_dev,
+                                     void *stream) {
+  if (!u || !out_dev || !h3d_grid_ok(g)) return H3D_EARG;
+  const int parts = nparts_for((long long)g->ny * g->nz);
+  if (parts > NOALLOC_PARTS) return H3D_ELIMIT;
+  Stats *part = nullptr;
+  const int rc = noalloc_part(&part);
+  if (rc != H3D_OK) return rc;
+  hipStream_t s = h3d_stream(stream);
+  hipLaunchKernelGGL((stats_kernel<double>), dim3(parts), dim3(RB), 0, s, u, g->nx, g->ny,
+                     g->nz, g->sy, g->sz, part);
+  int lrc = h3d_launch_status();
+  if (lrc != H3D_OK) return lrc;
+  hipLaunchKernelGGL((fold_partials<Stats>), dim3(1), dim3(RB), 0, s, part, parts);
+  lrc = h3d_launch_status();
+  if (lrc != H3D_OK) return lrc;
+  hipLaunchKernelGGL(stats_emit, dim3(1), dim3(64), 0, s, part, out_dev);
+  return h3d_launch_status();
+}
+
 extern "C" int h3d_face_sum_max_f64(const double *kappa, const h3d_grid *g,
                                     double *out, void *stream) {
   if (!kappa || !out || !h3d_grid_ok(g)) return H3D_EARG;

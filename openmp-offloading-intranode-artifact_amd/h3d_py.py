@@ -78,6 +78,7 @@ SIGNATURES = {
     "h3d_unpack_face_f32": (_I, [_P, _P, _GP, _I, _P]),
     "h3d_stats_f64": (_I, [_P, _GP, _P, _P]),
     "h3d_stats_f32": (_I, [_P, _GP, _P, _P]),
+    "h3d_stats_f64_noalloc": (_I, [_P, _GP, _P, _P]),
     "h3d_compare_f64": (_I, [_P, _P, _GP, _D, _D, _P, _P]),
     "h3d_compare_f32": (_I, [_P, _P, _GP, _D, _D, _P, _P]),
     "h3d_face_sum_max_f64": (_I, [_P, _GP, _P, _P]),
