@@ -229,17 +229,39 @@ __host__ __device__ constexpr int pass_lds_floats(int KB, int RY, int W) {
   return (KB + 1) * (W * RY) * 64 + 2 * KB * W * 2 * 64;
 }
 
+// How far ahead the FP32 pass loads planes, and how many prefetch slots it
+// cycles through (the march loop is unrolled by that many iterations, so
+// every slot index is static).  16-wave tiles (one workgroup per CU, a
+// barrier per plane) hold two planes in flight where the registers allow
+// it without spilling (up to 3 steps and 4 rows per lane, residual in
+// registers): 3-step passes on 64 x 64 tiles ran 81-84% of the stream rate
+// against 77% with one plane in flight.  Unrolling by 4 alone helps the
+// 16-wave tiles too; with the FP16 residual field those tiles spill, and
+// the longer loop made that worse, so they keep the 2-slot loop, as do
+// the 8- and 4-wave tiles (two or more workgroups per CU hide the latency).
+__host__ __device__ constexpr int pass_prefetch(int KB, int RY, int W, int MODE) {
+  return (W == 16 && MODE < 2 && KB <= 3 && RY <= 4) ? 2 : 1;
+}
+__host__ __device__ constexpr int pass_slots(int KB, int RY, int W, int MODE) {
+  return (W == 16 && MODE < 2) ? 4 : 2;
+}
+
 template <int KB, int RY, int W, int MODE, bool BF16>
 struct Pass {
+  // past the range of every plane resource (planes hold at most 2^28
+  // elements, 1 GiB of floats)
+  static constexpr unsigned OUT_OF_RANGE = 0x80000000u;
   static constexpr int TY = W * RY;
   static constexpr int SLOT = TY * 64;  // floats per kappa slot
   // ---- per-lane state ----
   float A[KB][2][RY];        // u of level l: slot P = plane it-l-1, 1-P = it-l
   float E[KB][2][RY];        // residual of level l, same slots
   float FZ[KB + 1][RY];      // z flux above the last plane of level l
-  float pu[2][RY];           // prefetched u rows (slot per phase)
-  float pk[2][RY];           // prefetched kappa rows
-  unsigned short pc[2][RY];  // prefetched residual bits (COMP16)
+  static constexpr int PF = pass_prefetch(KB, RY, W, MODE);  // planes ahead
+  static constexpr int NS = pass_slots(KB, RY, W, MODE);     // prefetch slots
+  float pu[NS][RY];           // prefetched u rows (slot = iteration mod NS)
+  float pk[NS][RY];           // prefetched kappa rows
+  unsigned short pc[NS][RY];  // prefetched residual bits (COMP16)
 
   // ---- geometry ----
   const float *uin;
@@ -285,10 +307,12 @@ struct Pass {
     }
   }
 
-  // One march iteration at phase P (P alternates; slot roles swap).
-  template <int P>
+  // One march iteration at phase PH (iteration mod NS): the level slots
+  // alternate with P = PH mod 2, the prefetch slots cycle with PH.
+  template <int PH>
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
+    constexpr int P = PH & 1;
     // 1. kappa of plane it into its ring slot (the slot of it-KB-1, which no
     //    level reads any more).  Other waves read these rows from the next
     //    iteration on, after the barrier; this lane's own rows of plane it
@@ -296,19 +320,19 @@ struct Pass {
     {
       float *ks = kr + eslot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[P][r];
+      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[PH][r];
     }
     // 2. Level 0: the loaded plane becomes the newest slot of level 0.
     float kq1[RY];  // kappa of plane q+1 (this lane's rows)
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      A[0][1 - P][r] = pu[P][r];
-      if (MODE == 2) E[0][1 - P][r] = comp_decode(pc[P][r], pu[P][r]);
+      A[0][1 - P][r] = pu[PH][r];
+      if (MODE == 2) E[0][1 - P][r] = comp_decode(pc[PH][r], pu[PH][r]);
       else E[0][1 - P][r] = 0.0f;
-      kq1[r] = pk[P][r];
+      kq1[r] = pk[PH][r];
     }
-    // 3. Prefetch the next plane (clamped, so always a valid address).
-    load_plane(it + 1, 1 - P);
+    // 3. Prefetch the plane PF ahead (clamped, so always a valid address).
+    load_plane(it + PF, (PH + PF) % NS);
 
     // 4. Levels 1..KB at planes it-1 .. it-KB.
     const int wb = w > 0 ? w - 1 : 0;
@@ -366,13 +390,15 @@ struct Pass {
           // only by that cell, so it needs no mask.
           A[l][1 - P][r] = inside ? res : u;
           E[l][1 - P][r] = e2;
-        } else if (q_out) {
+        } else {
           // Stored cells are interior on every axis, so the last level needs
-          // no Dirichlet select; rows and lanes off the output region carry
-          // an out-of-range offset and the buffer range check drops them.
-          // The branch is uniform (q is the same for the whole workgroup).
-          st_f32(ro, s4[r], res);
-          if (MODE == 2) st_u16(rc, s2[r], comp_encode(e2, res));
+          // no Dirichlet select.  Every row issues its store: rows and lanes
+          // off the output region, and every lane on a warm-up plane of the
+          // chunk (q outside zb..ze), carry an out-of-range offset and the
+          // buffer range check drops the write.  No branch, so the whole
+          // iteration is one basic block for the scheduler.
+          st_f32(ro, q_out ? s4[r] : OUT_OF_RANGE, res);
+          if (MODE == 2) st_u16(rc, q_out ? s2[r] : OUT_OF_RANGE, comp_encode(e2, res));
         }
       }
 #pragma unroll
@@ -429,9 +455,7 @@ __device__ __forceinline__ void pass_f32_tile(
     st.o4[r] = e * 4u;
     st.o2[r] = e * 2u;
   }
-  // Planes are at most 2^28 elements (1 GiB of floats), so 2^31 is past
-  // the range of every plane resource.
-  constexpr unsigned OUT_OF_RANGE = 0x80000000u;
+  constexpr unsigned OUT_OF_RANGE = S::OUT_OF_RANGE;
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r, tr = st.w * RY + r;
@@ -463,14 +487,20 @@ __device__ __forceinline__ void pass_f32_tile(
   for (int x = threadIdx.x; x < pass_lds_floats(KB, RY, W); x += W * 64) lds[x] = 0.0f;
   __syncthreads();
 
-  // Iterations it0 .. it0 + 2*pairs - 1 cover zb-KB .. ze+KB; an extra
-  // trailing iteration (odd count) writes nothing (its plane is > ze).
+  // Iterations it0 .. cover zb-KB .. ze+KB in groups of NS; trailing
+  // iterations of the last group write nothing (their planes are > ze).
   const int it0 = st.zb - KB;
-  const int pairs = (st.ze + KB - it0 + 2) / 2;
-  st.load_plane(it0, 0);
-  for (int p = 0; p < pairs; ++p) {
-    st.template iter<0>(it0 + 2 * p);
-    st.template iter<1>(it0 + 2 * p + 1);
+  constexpr int NS = S::NS;
+  const int groups = (st.ze + KB - it0 + NS) / NS;
+#pragma unroll
+  for (int f = 0; f < S::PF; ++f) st.load_plane(it0 + f, f);
+  for (int p = 0; p < groups; ++p) {
+    st.template iter<0>(it0 + NS * p);
+    st.template iter<1>(it0 + NS * p + 1);
+    if constexpr (NS == 4) {
+      st.template iter<2>(it0 + NS * p + 2);
+      st.template iter<3>(it0 + NS * p + 3);
+    }
   }
 }
 
@@ -498,16 +528,30 @@ __host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W) {
   return (KB + 1) * (W * RY) * 64 + 2 * KB * W * 2 * 64;
 }
 
+// Planes loaded ahead and prefetch slots of the FP64 pass (as
+// pass_prefetch / pass_slots for the FP32 pass).
+__host__ __device__ constexpr int pass64_prefetch(int KB, int RY, int W, bool RHS) {
+  return 1;
+}
+__host__ __device__ constexpr int pass64_slots(int KB, int RY, int W, bool RHS) {
+  return 2;
+}
+
 template <int KB, int RY, int W, bool RHS>
 struct Pass64 {
+  // at or past the range of every plane resource (planes hold at most
+  // 2^28 elements, 2 GiB of doubles)
+  static constexpr unsigned OUT_OF_RANGE = 0x80000000u;
   static constexpr int TY = W * RY;
   static constexpr int SLOT = TY * 64;  // doubles per kappa slot
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
-  double pu[2][RY];       // prefetched u rows (slot per phase)
-  double pk[2][RY];       // prefetched kappa rows
-  double pr[2][RY];       // prefetched rhs rows (RHS)
+  static constexpr int PF = pass64_prefetch(KB, RY, W, RHS);  // planes ahead
+  static constexpr int NS = pass64_slots(KB, RY, W, RHS);     // prefetch slots
+  double pu[NS][RY];      // prefetched u rows (slot = iteration mod NS)
+  double pk[NS][RY];      // prefetched kappa rows
+  double pr[NS][RY];      // prefetched rhs rows (RHS)
   double RH[KB][RY];      // rhs of planes it-1 .. it-KB (RHS; after the shift)
 
   // ---- geometry ----
@@ -550,10 +594,12 @@ struct Pass64 {
     }
   }
 
-  // One march iteration at phase P (P alternates; slot roles swap).
-  template <int P>
+  // One march iteration at phase PH (iteration mod NS): the level slots
+  // alternate with P = PH mod 2, the prefetch slots cycle with PH.
+  template <int PH>
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
+    constexpr int P = PH & 1;
     // 1. kappa of plane it into its ring slot (the slot of it-KB-1, which
     //    no level reads any more).  Other waves read these rows from the
     //    next iteration on, after the barrier; this lane's own rows of
@@ -561,20 +607,20 @@ struct Pass64 {
     {
       double *ks = kr + eslot(it) * SLOT;
 #pragma unroll
-      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[P][r];
+      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[PH][r];
     }
     double rn[RY];
 #pragma unroll
-    for (int r = 0; r < RY; ++r) rn[r] = RHS ? pr[P][r] : 0.0;  // rhs of plane it
+    for (int r = 0; r < RY; ++r) rn[r] = RHS ? pr[PH][r] : 0.0;  // rhs of plane it
     // 2. Level 0: the loaded plane becomes the newest slot of level 0.
     double kq1[RY];  // kappa of plane q+1 (this lane's rows)
 #pragma unroll
     for (int r = 0; r < RY; ++r) {
-      A[0][1 - P][r] = pu[P][r];
-      kq1[r] = pk[P][r];
+      A[0][1 - P][r] = pu[PH][r];
+      kq1[r] = pk[PH][r];
     }
-    // 3. Prefetch the next plane (clamped, so always a valid address).
-    load_plane(it + 1, 1 - P);
+    // 3. Prefetch the plane PF ahead (clamped, so always a valid address).
+    load_plane(it + PF, (PH + PF) % NS);
 
     // 4. Levels 1..KB at planes it-1 .. it-KB.
     const int wb = w > 0 ? w - 1 : 0;
@@ -619,11 +665,11 @@ struct Pass64 {
           // Dirichlet cells keep their value
           const bool inside = lane_int && row_int[r] && q_int;
           A[l][1 - P][r] = inside ? res : u;
-        } else if (q_out) {
+        } else {
           // stored cells are interior on every axis; rows and lanes off the
-          // output region carry an out-of-range offset (dropped write); the
-          // branch is uniform over the workgroup
-          st_f64(ro, s8[r], res);
+          // output region, and every lane on a warm-up plane of the chunk,
+          // carry an out-of-range offset (dropped write), so no branch
+          st_f64(ro, q_out ? s8[r] : OUT_OF_RANGE, res);
         }
       }
 #pragma unroll
@@ -685,9 +731,7 @@ __device__ __forceinline__ void pass_f64_tile(
     const int jc = j < 0 ? 0 : (j > ny - 1 ? ny - 1 : j);
     st.o8[r] = (unsigned)(jc * sy + ic) * 8u;
   }
-  // Planes are at most 2^28 elements (2 GiB of doubles), so 2^31 bytes is
-  // at or past the range of every plane resource.
-  constexpr unsigned OUT_OF_RANGE = 0x80000000u;
+  constexpr unsigned OUT_OF_RANGE = S::OUT_OF_RANGE;
 #pragma unroll
   for (int r = 0; r < RY; ++r) {
     const int j = j0 + r, tr = st.w * RY + r;
@@ -717,14 +761,20 @@ __device__ __forceinline__ void pass_f64_tile(
   for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W); x += W * 64) lds[x] = 0.0;
   __syncthreads();
 
-  // Iterations it0 .. it0 + 2*pairs - 1 cover zb-KB .. ze+KB; an extra
-  // trailing iteration (odd count) writes nothing (its plane is > ze).
+  // Iterations it0 .. cover zb-KB .. ze+KB in groups of NS; trailing
+  // iterations of the last group write nothing (their planes are > ze).
   const int it0 = st.zb - KB;
-  const int pairs = (st.ze + KB - it0 + 2) / 2;
-  st.load_plane(it0, 0);
-  for (int p = 0; p < pairs; ++p) {
-    st.template iter<0>(it0 + 2 * p);
-    st.template iter<1>(it0 + 2 * p + 1);
+  constexpr int NS = S::NS;
+  const int groups = (st.ze + KB - it0 + NS) / NS;
+#pragma unroll
+  for (int f = 0; f < S::PF; ++f) st.load_plane(it0 + f, f);
+  for (int p = 0; p < groups; ++p) {
+    st.template iter<0>(it0 + NS * p);
+    st.template iter<1>(it0 + NS * p + 1);
+    if constexpr (NS == 4) {
+      st.template iter<2>(it0 + NS * p + 2);
+      st.template iter<3>(it0 + NS * p + 3);
+    }
   }
 }
 
