@@ -22,6 +22,7 @@ constexpr int MAXPART = 1024; // partial records
 struct Ksum {  // Neumaier pair
   double s, c;
   __device__ void add(double x) {
+#pragma clang fp contract(off)
     double t = s + x;
     if (fabs(s) >= fabs(x)) c += (s - t) + x;
     else c += (x - t) + s;
@@ -41,6 +42,7 @@ struct Stats {
   double mx, bad;
   __device__ void init() { sum = {0, 0}; sq = {0, 0}; mx = 0; bad = 0; }
   __device__ void take(double v) {
+#pragma clang fp contract(off)
     if (!isfinite(v)) { bad += 1.0; return; }
     sum.add(v);
     sq.add(v * v);
@@ -74,6 +76,10 @@ struct Cmp {
     mx = -1.0; imx = 1e300; mrel = 0; nnan = 0; ninf = 0; nfail = 0; relnan = 0;
   }
   __device__ void take(double r, double c, double idx, double atol, double rtol) {
+    // numpy's unfused arithmetic: |d| <= atol + rtol*|ref| rounds the
+    // product before the sum, and d*d, r*r are rounded before they are
+    // summed (the file is also built without contraction)
+#pragma clang fp contract(off)
     if (isnan(c)) nnan += 1.0;
     else if (isinf(c)) ninf += 1.0;
     const double d = c - r, ad = fabs(d), ar = fabs(r);

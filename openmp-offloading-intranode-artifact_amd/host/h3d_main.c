@@ -22,7 +22,7 @@
  *                      FP64 plain (default); FP32 storage plain; FP32
  *                      compensated with register carry; with FP16 residual
  *   --kappa-bf16       store kappa in BF16 with a per-field scale (FP32)
- *   --kblock K         time steps fused per HBM pass (FP32 1..6, FP64 1..4;
+ *   --kblock K         time steps fused per HBM pass (FP32 1..8, FP64 1..4;
  *                      the results are the same bits as single steps)
  *   --tile R,W,Z       rows per lane, waves per workgroup, planes per chunk
  *   --autotune         time candidate launch shapes first and use the best

@@ -104,6 +104,70 @@ def test_cli_deep_slab_runs_with_band_exchange(tmp_path, mode, part):
     assert np.array_equal(v[0:2], v[8:10]) and np.array_equal(v[10:12], v[2:4])
 
 
+@pytest.mark.parametrize("prec,kblock", [("f64", 4), ("f32c", 5)])
+def test_cli_fixed_depth_is_autotuned_on_large_grids(prec, kblock, tmp_path):
+    """A --kblock the library accepts is searched at that depth when the
+    program autotunes by default (from 128^3 interior points), and every
+    shape it may pick gives the bits of the default shape."""
+    args = ["130", "--steps", "11", "--mode", "seeded", "--precision", prec,
+            "--kblock", str(kblock)]
+    tuned = run(args, str(tmp_path), write=False)
+    assert re.search(rf"^Autotuned: .*kblock={kblock}$", tuned, re.M), tuned
+    plain = run(args + ["--no-autotune"], str(tmp_path), write=False)
+    line = lambda out: next(ln for ln in out.splitlines() if ln.startswith("checksum ="))
+    assert line(tuned) == line(plain)
+
+
+def cli_deep_slab_initial(gpu, n, parts, part, ghost, seeded):
+    """The initial slab the program builds for --slab parts:part with
+    --kblock ghost (host/h3d_main.c): the reference's partition init with
+    `ghost` ghost planes, or the seeded generators on the planes inside the
+    domain with the nearest domain plane repeated beyond it; and the
+    step's factor (seeded: from the global kappa bound)."""
+    rc, nzd, gks = gpu.slab_decompose(n, parts)
+    assert rc == 0
+    nz = nzd[part] - 2 + 2 * ghost
+    gk0 = gks[part] - ghost
+    g = gpu.Grid.dense(n, n, nz)
+    P = n * n
+    if not seeded:
+        h, dt, factor = gpu.params_reference(n)
+        u, rhs, k = (np.zeros(nz * P) for _ in range(3))
+        gpu.call("h3d_init_partition", gpu.ptr(u), gpu.ptr(rhs), gpu.ptr(k), nz, gk0 + ghost,
+                 ghost, n, n, n, h, dt)
+        return g, u, rhs, k, factor
+    lo, hi = max(gk0, 0), min(gk0 + nz, n)
+    gin = gpu.Grid.dense(n, n, hi - lo)
+    ui, ki = gpu.gen_seeded(gin, lo, n, n, n, 0, 20260101)
+    u, k = np.zeros(nz * P), np.zeros(nz * P)
+    for z in range(nz):
+        src = min(max(z + gk0, lo), hi - 1) - lo
+        u[z * P:(z + 1) * P] = ui[src * P:(src + 1) * P]
+        k[z * P:(z + 1) * P] = ki[src * P:(src + 1) * P]
+    _, kg = gpu.gen_seeded(gpu.Grid.dense(n), 0, n, n, n, 0, 20260101)
+    h = 1.0 / (n - 1)
+    fmax = gpu.lib().h3d_face_sum_max_host(gpu.ptr(kg), gpu.C.byref(gpu.Grid.dense(n)))
+    dt = 0.9 * 2.0 * h * h / fmax
+    return g, u, None, k, dt / h / h / 2.0
+
+
+@pytest.mark.parametrize("mode,part", [("reference", 1), ("seeded", 0), ("seeded", 3)])
+def test_cli_deep_slab_equals_oracle(gpu, oracle, tmp_path, mode, part):
+    """The program's deep-ghost slab (--slab 4:p --kblock 2) ends on the
+    bits of the oracle stepping the same initial slab -- built on the host
+    as the program builds it -- with the same band copies after every
+    2-step pass."""
+    from test_gpu_face_reduce import deep_standin_oracle
+    n, ghost = 34, 2
+    out = run([str(n), "--steps", "5", "--mode", mode, "--slab", f"4:{part}", "--kblock",
+               str(ghost)], str(tmp_path))
+    steps = int(re.search(r"^(\d+) timesteps complete$", out, re.M).group(1))
+    g, u, rhs, k, factor = cli_deep_slab_initial(gpu, n, 4, part, ghost, mode == "seeded")
+    want = deep_standin_oracle(oracle, u, rhs, k, g, factor, ghost, steps)
+    got = np.loadtxt(tmp_path / f"u_h3d_slab{part}of4.txt")
+    assert got.size == want.size and np.array_equal(got, want)
+
+
 def test_cli_autotune(tmp_path):
     out = run(["70", "--steps", "4", "--mode", "seeded", "--autotune"], str(tmp_path),
               write=False)

@@ -413,3 +413,31 @@ def test_device_copy_completes_before_return(gpu):
         s.sync()
         outs.append(b.download())
     assert outs[0].tobytes() == outs[1].tobytes()
+
+
+def test_compare_pass_rule_at_the_tolerance_boundary(gpu, oracle):
+    """The PASS rule |c - r| <= atol + rtol*|r| decided on the device as
+    numpy decides it (compare.py:65-76, unfused arithmetic), on candidates
+    one step below, at and one step above the rounded r + bound.  With
+    rtol = 0.3 the bound's spacing is close to that of c - r (about a
+    tenth of the cells sit exactly on the bound), so a fused multiply-add
+    in the bound (one rounding instead of two) changes the decision on
+    thousands of these cells; the failure count must equal numpy's and
+    the oracle's."""
+    rng = np.random.default_rng(17)
+    atol, rtol = 1e-6, 0.3
+    r = rng.uniform(1.0, 2.0, 30_000) * rng.choice([-1.0, 1.0], 30_000)
+    c = r + (atol + rtol * np.abs(r))
+    ref = np.concatenate([r, r, r])
+    cand = np.concatenate([np.nextafter(c, -np.inf), c, np.nextafter(c, np.inf)])
+    ad, bound = np.abs(cand - ref), atol + rtol * np.abs(ref)
+    want = int(np.sum(~(ad <= bound)))
+    assert int(np.sum(ad == bound)) > 1000 and 0 < want < ref.size
+    # a 3 x 3 x nx block holds them (every point of the array is compared)
+    nx = ref.size // 9
+    assert 9 * nx == ref.size
+    g = gpu.Grid(nx, 3, 3, nx, 3 * nx)
+    dr, dc = gpu.DeviceArray.from_host(ref), gpu.DeviceArray.from_host(cand)
+    got = gpu.compare(dr, dc, g, atol=atol, rtol=rtol)
+    assert int(got[8]) == want
+    assert int(oracle.metrics(ref, cand, atol, rtol)[8]) == want
