@@ -70,6 +70,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-transfer", action="store_true",
                     help="skip the host <-> device transfer sweep")
+    ap.add_argument("--config-steps", type=int, default=0,
+                    help="steps of each configuration leg (0: the configuration's own; "
+                         "profiling runs use a few, since counters are per launch)")
+    ap.add_argument("--only", default="",
+                    help="comma list of configuration legs to run: reference,slab,cfg4,ensemble")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--dense", action="store_true",
                     help="reference's dense layout instead of 128-byte aligned rows")
@@ -84,26 +89,41 @@ def parse():
     return ap.parse_args()
 
 
-def profile_entry(key):
+def profile_entry(key, alg_bytes=None):
     """The rocprofv3 figures committed under profiles/ for one kernel and
     launch shape (key "kernel<template args>|grid threads|workgroup"):
     HBM bytes per launch from FETCH_SIZE / WRITE_SIZE scaled by the
     copy-kernel calibration, and the kernel trace's average duration.
-    None when that shape was not profiled."""
+
+    When that exact shape was not profiled (the autotuner may pick another
+    z chunk from run to run) and alg_bytes is given, the profiled launch of
+    the same kernel instantiation, workgroup size and problem (equal
+    algorithmic bytes) with the nearest grid stands in; its key is returned
+    as "profiled_launch".  None when neither exists."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     try:
-        return json.load(open(path)).get(key)
+        table = json.load(open(path))
     except (OSError, ValueError):
         return None
+    if alg_bytes is None or "|" not in key:
+        return dict(table[key], profiled_launch=key) if key in table else None
+    kern, grid, wg = key.split("|")
+    near = [(abs(int(k.split("|")[1]) - int(grid)), k) for k, e in table.items()
+            if k.count("|") == 2 and k.split("|")[0] == kern and k.split("|")[2] == wg
+            and e.get("algorithmic_bytes_per_launch") == int(alg_bytes)]
+    if not near:
+        return None
+    k = min(near)[1]
+    return dict(table[k], profiled_launch=k)
 
 
 def traffic_ratio(key, alg_bytes):
-    e = profile_entry(key)
+    e = profile_entry(key, alg_bytes)
     if not e or not e.get("bytes_per_launch"):
         return None
     return {"bytes_per_launch": e["bytes_per_launch"],
             "ratio_to_algorithmic": round(e["bytes_per_launch"] / alg_bytes, 4),
-            "profile": e.get("source")}
+            "profiled_launch": e["profiled_launch"], "profile": e.get("source")}
 
 
 def pass_key(kernel, kb, rows, waves, zc, m, extra=""):
@@ -113,6 +133,27 @@ def pass_key(kernel, kb, rows, waves, zc, m, extra=""):
     tiles = -(-m // ox) * -(-m // oy)
     wgs = tiles * -(-m // zc)
     return f"{kernel}<{kb}, {rows}, {waves}{extra}>|{wgs * 64 * waves}|{64 * waves}"
+
+
+def march_key(kernel, rows, waves, zc, nx, ny, planes, extra):
+    """Profile key of a single-step (march) launch on a field whose first
+    interior point starts a 128-byte line: 64-column tiles from x = 1."""
+    tiles = -(-(nx - 2) // 64) * -(-(ny - 2) // (rows * waves))
+    wgs = tiles * -(-planes // zc)
+    return f"{kernel}<{rows}{extra}>|{wgs * 64 * waves}|{64 * waves}"
+
+
+def note_launch(label, key, alg_bytes):
+    """Logs the profile key and algorithmic bytes of a leg's dominant
+    launch (profiling runs read these lines to label the counters)."""
+    log(f"launch {label} key={key} alg_bytes={int(alg_bytes)}")
+
+
+def config_steps(own):
+    return ARGS.config_steps if ARGS and ARGS.config_steps > 0 else own
+
+
+ARGS = None
 
 
 def median(xs):
@@ -251,7 +292,7 @@ def leg_reference_workload(bw, stream):
     written)."""
     n = 512
     h, dt, factor = h3d.params_reference(n)
-    T = float("%.7g" % (500 * dt))
+    T = float("%.7g" % (config_steps(500) * dt))
     steps = h3d.count_steps(T, dt)
     g = h3d.pitched_grid(n, n, n)
     u, r, k = (np.zeros(g.elems) for _ in range(3))
@@ -278,12 +319,19 @@ def leg_reference_workload(bw, stream):
         d.close()
     mi250x_hip = 13.06  # GLUP/s, 1 GCD, HIP/summary.csv:2 (5.0899 s for 501 steps)
     bpl = 32.0 / max(1, t.kblock)  # u, rhs, kappa read and u written once per pass
+    m = n - 2
+    if t.kblock > 1:
+        key = pass_key("h3d_pass_f64_kernel", t.kblock, t.rows, t.waves, t.zchunk, m, ", true")
+    else:
+        key = march_key("h3d_march_f64", t.rows, t.waves, t.zchunk, n, n, m, ", true")
+    note_launch("reference_workload", key, 32.0 * m ** 3)
     return {"value": round(glups, 3), "unit": "GLUP/s", "dtype": "f64", "n": n,
             "steps": steps, "T": "%.7g" % T, "seconds": round(sec, 5),
             "algorithmic_bytes_per_lup": bpl,
             "achieved_gbs": round(bpl * lups / sec / 1e9, 1),
             "frac_of_stream": round(bpl * lups / sec / 1e9 / bw, 4),
             "tune": [t.rows, t.waves, t.zchunk], "kblock": t.kblock, "checksum": float(st[0]),
+            "launch": key, "traffic": traffic_ratio(key, 32.0 * m ** 3),
             "reference_mi250x_hip_glups": mi250x_hip,
             "ratio_to_reference_mi250x_hip": round(glups / mi250x_hip, 2)}
 
@@ -293,7 +341,7 @@ def leg_cfg3(bw, stream):
     planes + 2 ghosts (1026 x 1026 x 258 points), lognormal kappa, 500 FP64
     steps, boundary planes and in-HBM face pack/unpack overlapped with the
     interior, replayed from a hipGraph.  dt from this slab's kappa."""
-    n, planes, steps = 1026, 258, 500
+    n, planes, steps = 1026, 258, config_steps(500)
     rc, nzd, gks = h3d.slab_decompose(n, 4)
     assert rc == 0 and nzd[1] == planes
     g = h3d.pitched_grid(n, n, planes)
@@ -323,12 +371,16 @@ def leg_cfg3(bw, stream):
         f.close()
     deep = {f"ghost{K}": slab_deep(n, planes - 2, gks[1], K, factor, steps, bw, stream)
             for K in (2, 3)}
+    # the interior launch (planes 2..nz-3) dominates a step
+    key = march_key("h3d_march_f64", t.rows, t.waves, t.zchunk, n, n, planes - 4, ", false")
+    alg = 24.0 * (n - 2) ** 2 * (planes - 4)
+    note_launch("slab_ghost1_interior", key, alg)
     return {"value": round(lups / (ms.value / 1e3) / 1e9, 3), "unit": "GLUP/s",
             "dtype": "f64", "steps": steps, "ms_per_step": round(ms.value / steps, 5),
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
             "slab": [n, n, planes], "nz_split": nzd, "face_bytes": n * n * 8,
             "tune": [t.rows, t.waves, t.zchunk],
-            "nonfinite": float(st[3]),
+            "nonfinite": float(st[3]), "launch": key, "traffic": traffic_ratio(key, alg),
             "fused_passes_deep_ghosts": deep}
 
 
@@ -365,11 +417,18 @@ def slab_deep(n, owned, gk_owned, K, factor, steps, bw, stream):
     st = h3d.stats(db if fb.value else da, g)
     for f in fields + [lo, hi]:
         f.close()
+    # the interior pass (planes 2K..nz-2K-1) dominates a pass
+    key = pass_key("h3d_pass_f64_kernel", K, t.rows, t.waves, t.zchunk, n - 2, ", false")
+    key = key.rsplit("|", 2)[0] + "|" + str(
+        -(-(n - 2) // (64 - 2 * K)) * -(-(n - 2) // (t.rows * t.waves - 2 * K))
+        * -(-(planes - 4 * K) // t.zchunk) * 64 * t.waves) + "|" + str(64 * t.waves)
+    alg = 24.0 * (n - 2) ** 2 * (planes - 4 * K)
+    note_launch(f"slab_ghost{K}_interior", key, alg)
     return {"glups": round(lups / (ms.value / 1e3) / 1e9, 3),
             "ms_per_step": round(ms.value / steps, 5), "achieved_gbs": round(gbs, 1),
             "frac_of_stream": round(gbs / bw, 4), "slab": [n, n, planes],
             "band_bytes": K * n * n * 8, "tune": [t.rows, t.waves, t.zchunk, t.kblock],
-            "nonfinite": float(st[3])}
+            "nonfinite": float(st[3]), "launch": key, "traffic": traffic_ratio(key, alg)}
 
 
 def leg_cfg4(bw, stream):
@@ -381,7 +440,7 @@ def leg_cfg4(bw, stream):
       - the temporal-blocking depths 1, 2, 4 and 8 (FP32 kappa, residual in
         registers), each on its own tuned shape.
     Every run starts from a device-to-device copy of the initial field."""
-    m, steps = 1024, 2000
+    m, steps = 1024, config_steps(2000)
     n = m + 2
     g, u, k, factor = seeded_problem(n, n, n, 0, 20260101)
     u32 = u.astype(np.float32)
@@ -445,11 +504,14 @@ def leg_cfg4(bw, stream):
              "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
              "rel_l2_vs_fp64": float(mt[3]), "max_abs_vs_fp64": float(mt[0]),
              "max_rel_vs_fp64": float(mt[4]), "graph": True}
+        extra = f", {mode}, {'true' if kb16 is not None else 'false'}"
         if t.kblock > 1:
-            extra = f", {mode}, {'true' if kb16 is not None else 'false'}"
             key = pass_key("h3d_pass_f32_kernel", t.kblock, t.rows, t.waves, t.zchunk, m, extra)
-            r["launch"] = key
-            r["traffic"] = traffic_ratio(key, bpl * m ** 3)
+        else:
+            key = march_key("h3d_march_f32", t.rows, t.waves, t.zchunk, n, n, m, extra)
+        r["launch"] = key
+        r["traffic"] = traffic_ratio(key, bpl * m ** 3)
+        note_launch(f"cfg4_{label}", key, bpl * m ** 3)
         log(f"cfg4 {label}: {r}")
         return r
 
@@ -458,6 +520,12 @@ def leg_cfg4(bw, stream):
     ref, ms64, t64 = f64_reference(k32.astype(np.float64))
     out["f32_kappa_fp64_reference_glups"] = round(m ** 3 * steps / (ms64 / 1e3) / 1e9, 3)
     out["f32_kappa_fp64_reference_tune"] = [t64.rows, t64.waves, t64.zchunk, t64.kblock]
+    if t64.kblock > 1:
+        k64 = pass_key("h3d_pass_f64_kernel", t64.kblock, t64.rows, t64.waves, t64.zchunk, m,
+                       ", false")
+        out["f32_kappa_fp64_reference_launch"] = k64
+        out["f32_kappa_fp64_reference_traffic"] = traffic_ratio(k64, 24.0 * m ** 3)
+        note_launch("cfg4_fp64_reference", k64, 24.0 * m ** 3)
     f32f = h3d.C.c_float(np.float32(factor))
     out["f32_kappa_carry"] = leg("f32_kappa_carry", fk32, fk32.ptr, None, f32f, 1, 0, ref, 4.0)
     out["f32_kappa_fp16_residual"] = leg("f32_kappa_fp16_residual", fk32, fk32.ptr, None,
@@ -522,16 +590,17 @@ def leg_cfg5(bw, stream):
     ms = h3d.C.c_float()
     et = h3d.Tune()
     h3d.call("h3d_ensemble_autotune", e, stream.ptr, h3d.C.byref(et))  # z-march depth
-    h3d.call("h3d_ensemble_run", e, 500, 1, stream.ptr, h3d.C.byref(ms))  # builds the graph
-    h3d.call("h3d_ensemble_run", e, 500, 1, stream.ptr, h3d.C.byref(ms))  # timed: 1000 total
+    steps = config_steps(500)
+    h3d.call("h3d_ensemble_run", e, steps, 1, stream.ptr, h3d.C.byref(ms))  # builds the graph
+    h3d.call("h3d_ensemble_run", e, steps, 1, stream.ptr, h3d.C.byref(ms))  # timed
     nwork, nblk, done = h3d.C.c_int(), h3d.C.c_int(), h3d.C.c_longlong()
     h3d.call("h3d_ensemble_info", e, h3d.C.byref(nwork), h3d.C.byref(nblk), h3d.C.byref(done))
     cells = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in dims)
     sec = ms.value / 1e3
-    glups = cells * 500 / sec / 1e9
+    glups = cells * steps / sec / 1e9
     twins = [dict(s, precision=0) for s in specs[1::2]]
     e2, keep2 = build(twins)
-    h3d.call("h3d_ensemble_run", e2, 1000, 1, stream.ptr, h3d.C.byref(ms))
+    h3d.call("h3d_ensemble_run", e2, 2 * steps, 1, stream.ptr, h3d.C.byref(ms))
     per_member = []
     in_b = h3d.C.c_int()
     for mi, ((da, db, dk), (ta, tb, tk)) in enumerate(zip(keep[1::2], keep2)):
@@ -554,11 +623,22 @@ def leg_cfg5(bw, stream):
     k64 = et.kblock
     b64 = sum(24.0 / k64 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
     b32 = sum(16.0 / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
-    gbs = (b64 + b32) * 500 / sec / 1e9
+    gbs = (b64 + b32) * steps / sec / 1e9
+    # one launch per precision and group of kblock steps: the pass over all
+    # members of that precision (profile keys by kernel name: the grid is
+    # the device's resident workgroups)
+    c64 = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
+    c32 = cells - c64
+    traffic = {}
+    for prec, alg in ((0, 24.0 * c64), (1, 16.0 * c32)):
+        key = f"ensemble_step<{et.kblock}, {prec}>"
+        note_launch(f"ensemble_prec{prec}", key, alg)
+        traffic["fp64_members" if prec == 0 else "fp32_members"] = {
+            "launch": key, "traffic": traffic_ratio(key, alg)}
     return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
-            "cells": int(cells), "steps_timed": 500, "graph": True, "zchunk": et.zchunk,
-            "kblock": et.kblock,
-            "ms_per_step": round(1e3 * sec / 500, 5),
+            "cells": int(cells), "steps_timed": steps, "graph": True, "zchunk": et.zchunk,
+            "kblock": et.kblock, "launches": traffic,
+            "ms_per_step": round(1e3 * sec / steps, 5),
             "work_items": nwork.value, "workgroups": nblk.value,
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
             "worst_fp32_member_rel_l2_vs_fp64": worst,
@@ -566,7 +646,9 @@ def leg_cfg5(bw, stream):
 
 
 def main():
+    global ARGS
     a = parse()
+    ARGS = a
     if a.gpus != 1:
         print(f"bench.py: one GPU only (--gpus {a.gpus} requested): the update would "
               "shard as z-slabs, one per GPU, exchanging one face per neighbour per "
@@ -669,7 +751,6 @@ def main():
     else:
         tune1 = tune if kb64 == 1 else tuned64(1)
     glups1, wall1, per_launch1, _, walls1 = fp64_leg(tune1)
-    key1 = None  # the single-step tile's grid depends on the field's alignment
     single = {"value": round(glups1, 3), "unit": "GLUP/s", "kblock": 1,
               "tune": {"rows": tune1.rows, "waves": tune1.waves, "zchunk": tune1.zchunk},
               "ms_per_step": round(1e3 * wall1 / a.steps, 5), "repeats": len(walls1),
@@ -683,13 +764,20 @@ def main():
         kname = "h3d_pass_f64_kernel"
         key = pass_key(kname, kb64, tune.rows, tune.waves, tune.zchunk, m, ", false")
     else:
-        kname, key = "h3d_march_f64", None
-    prof = profile_entry(key) if key else None
+        kname = "h3d_march_f64"
+        key = march_key(kname, tune.rows, tune.waves, tune.zchunk, n, n, m, ", false")
+    note_launch("fp64_headline", key, alg_bytes)
+    key1 = march_key("h3d_march_f64", tune1.rows, tune1.waves, tune1.zchunk, n, n, m, ", false")
+    note_launch("fp64_single_step", key1, alg_bytes)
+    single["launch"] = key1
+    single["traffic"] = traffic_ratio(key1, alg_bytes)
+    prof = profile_entry(key, alg_bytes) if key else None
     profiled = None
     if prof and prof.get("trace_avg_us"):
         pa = alg_bytes / (prof["trace_avg_us"] * 1e-6) / 1e9
         profiled = {"kernel_us_per_launch": prof["trace_avg_us"], "achieved": round(pa, 1),
-                    "frac": round(pa / HBM_PEAK_GBS, 4), "profile": prof.get("source")}
+                    "frac": round(pa / HBM_PEAK_GBS, 4), "launch": prof["profiled_launch"],
+                    "profile": prof.get("source")}
 
     result = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUP/s", "n_gpus": 1,
@@ -709,8 +797,8 @@ def main():
                    "seconds_each": [round(w, 5) for w in walls]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": (profile_entry(key) or {}).get("bytes_per_launch") if key
-                     else None,
+                     "traffic": (profile_entry(key, alg_bytes) or {}).get("bytes_per_launch")
+                     if key else None,
                      "kernel": kname, "launch": key, "steps_per_launch": kb64,
                      "frac_of_stream": round(achieved / bw, 4),
                      "algorithmic_bytes_per_launch": alg_bytes,
@@ -782,9 +870,14 @@ def main():
             d2d(fbuf, f0)
             (res, _), _ = run32(fa, fbuf, acc_steps, t32)
             m32 = h3d.compare(ref, res, g, f32=True)
-            key32 = pass_key("h3d_pass_f32_kernel", kb, t32.rows, t32.waves, t32.zchunk, m,
-                             ", 1, false") if kb > 1 else None
-            prof32 = profile_entry(key32) if key32 else None
+            if kb > 1:
+                key32 = pass_key("h3d_pass_f32_kernel", kb, t32.rows, t32.waves, t32.zchunk, m,
+                                 ", 1, false")
+            else:
+                key32 = march_key("h3d_march_f32", t32.rows, t32.waves, t32.zchunk, n, n, m,
+                                  ", 1, false")
+            note_launch(f"fp32_k{kb}", key32, alg32)
+            prof32 = profile_entry(key32, alg32)
             log(f"FP32c k={kb}: {glups32:.1f} GLUP/s, {gbs32:.0f} GB/s "
                 f"({100 * gbs32 / bw:.1f}% of stream), rel-L2 {m32[3]:.2e} after "
                 f"{acc_steps} steps, max-abs {m32[0]:.2e}")
@@ -806,8 +899,7 @@ def main():
                 "accuracy_steps": acc_steps,
                 "rel_l2_vs_fp64": float(m32[3]), "max_abs_vs_fp64": float(m32[0]),
                 "max_rel_vs_fp64": float(m32[4]),
-                "launch": key32,
-                "traffic": traffic_ratio(key32, alg32) if key32 else None,
+                "launch": key32, "traffic": traffic_ratio(key32, alg32),
             }
             if prof32 and prof32.get("trace_avg_us"):
                 out["profiled_frac_of_stream"] = round(
@@ -842,10 +934,14 @@ def main():
         d.close()
     if not a.no_configs:
         result["configs"] = {}
-        for name, fn in (("reference_workload_n512_501steps", leg_reference_workload),
-                         ("slab_1024x1024x256_fp64", leg_cfg3),
-                         ("fp32c_1024cubed_2000steps", leg_cfg4),
-                         ("ensemble_64_members_500steps", leg_cfg5)):
+        only = set(x for x in a.only.split(",") if x)
+        for short, name, fn in (("reference", "reference_workload_n512_501steps",
+                                 leg_reference_workload),
+                                ("slab", "slab_1024x1024x256_fp64", leg_cfg3),
+                                ("cfg4", "fp32c_1024cubed_2000steps", leg_cfg4),
+                                ("ensemble", "ensemble_64_members_500steps", leg_cfg5)):
+            if only and short not in only:
+                continue
             log(f"{name} ...")
             try:
                 result["configs"][name] = fn(bw, stream)

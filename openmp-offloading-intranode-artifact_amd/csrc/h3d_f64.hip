@@ -105,10 +105,12 @@ h3d_pass_f64_kernel(double *__restrict__ uout, const double *__restrict__ uin,
                                 zb, min(ke, zb + zchunk - 1), tx, ty);
 }
 
+// With `occ` set nothing is launched: the shape's resident workgroups per
+// CU go to *occ and its tiles per z chunk to *tiles.
 template <int KB, int RY, int W, bool RHS>
 int launch_pass_one(double *uo, const double *ui, const double *rhs, const double *kf,
                     const h3d_grid *g, double factor, int ks, int ke, int zchunk,
-                    hipStream_t s) {
+                    hipStream_t s, int *occ, long long *tiles) {
   constexpr int OX = 64 - 2 * KB, OY = W * RY - 2 * KB;
   if (OY < 1) return H3D_EARG;
   constexpr size_t lds = sizeof(double) * (size_t)pass64_lds_doubles(KB, RY, W);
@@ -120,6 +122,18 @@ int launch_pass_one(double *uo, const double *ui, const double *rhs, const doubl
     static std::atomic<unsigned long long> big_lds_done{0};
     const int big_rc = h3d_allow_big_lds((const void *)fn, &big_lds_done);
     if (big_rc != H3D_OK) return big_rc;
+  }
+  if (occ) {
+    // resident workgroups per CU depend on the kernel alone: asked once
+    static std::atomic<int> occ_known{0};
+    int o = occ_known.load(std::memory_order_relaxed);
+    if (o == 0) {
+      H3D_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&o, fn, 64 * W, lds));
+      occ_known.store(o, std::memory_order_relaxed);
+    }
+    *occ = o;
+    if (tiles) *tiles = (long long)tiles_x * tiles_y;
+    return H3D_OK;
   }
   const long long nb =
       (long long)tiles_x * tiles_y * ((ke - ks + 1 + zchunk - 1) / zchunk);
@@ -133,13 +147,14 @@ int launch_pass_one(double *uo, const double *ui, const double *rhs, const doubl
 template <int KB>
 int launch_pass_kb(int ry, int waves, double *uo, const double *ui, const double *rhs,
                    const double *kf, const h3d_grid *g, double factor, int ks, int ke,
-                   int zchunk, hipStream_t s) {
+                   int zchunk, hipStream_t s, int *occ = nullptr,
+                   long long *tiles = nullptr) {
 #define H3D_P(RY, W)                                                                 \
   if (ry == RY && waves == W)                                                       \
     return rhs ? launch_pass_one<KB, RY, W, true>(uo, ui, rhs, kf, g, factor, ks, ke, \
-                                                  zchunk, s)                        \
+                                                  zchunk, s, occ, tiles)            \
                : launch_pass_one<KB, RY, W, false>(uo, ui, rhs, kf, g, factor, ks, ke, \
-                                                   zchunk, s);
+                                                   zchunk, s, occ, tiles);
   H3D_P(2, 4) H3D_P(2, 8) H3D_P(4, 4) H3D_P(4, 8)
   // 32 rows as 16 waves of 2: four waves per SIMD where rows 4 x 8 waves
   // run two (160 KB of LDS at 3 steps; 208 KB at 4 would not fit)
@@ -158,6 +173,19 @@ bool tune_ok(const h3d_tune &t) {
 }
 
 }  // namespace
+
+int h3d_pass_f64_geometry(const h3d_grid *g, int kblock, int rows, int waves, int rhs,
+                          int *wg_per_cu, long long *tiles) {
+  // a non-null stand-in for rhs selects the kernel that carries it; the
+  // query reads no memory
+  const double *r = rhs ? reinterpret_cast<const double *>(g) : nullptr;
+  switch (kblock) {
+    case 2: return launch_pass_kb<2>(rows, waves, nullptr, nullptr, r, nullptr, g, 0.0, 1, g->nz - 2, 1, nullptr, wg_per_cu, tiles);
+    case 3: return launch_pass_kb<3>(rows, waves, nullptr, nullptr, r, nullptr, g, 0.0, 1, g->nz - 2, 1, nullptr, wg_per_cu, tiles);
+    case 4: return launch_pass_kb<4>(rows, waves, nullptr, nullptr, r, nullptr, g, 0.0, 1, g->nz - 2, 1, nullptr, wg_per_cu, tiles);
+  }
+  return H3D_EARG;
+}
 
 int h3d_default_tune_f64(const h3d_grid *g, int k_start, int k_end,
                          h3d_tune *t) {
@@ -203,18 +231,26 @@ extern "C" int h3d_step_f64(double *u_new, const double *u_old,
   }
 }
 
-// Default shape of a fused pass: rows 2, waves 8 (rows 4, waves 4 at 4
-// steps, where 2 x 8 needs 180 KB of LDS), z chunks short enough for 1024
-// workgroups (256 CUs x 4) while the z warm-up (2*kblock planes per chunk)
-// stays small.
-static void default_tune_pass64(const h3d_grid *g, int kb, int planes, h3d_tune *t) {
+// Default shape of a fused pass: 32-row tiles of 16 waves x 2 rows at 2
+// and 3 steps (one 1024-thread workgroup per CU, the tuner's pick at
+// 512^3), rows 4 x waves 8 at 4 steps (16 waves x 2 rows would need 208 KB
+// of LDS); the z chunk that fills whole rounds of resident workgroups
+// (h3d_round_zchunks), or short enough for 1024 workgroups if the shape's
+// geometry cannot be asked.
+static void default_tune_pass64(const h3d_grid *g, int kb, int planes, int rhs, h3d_tune *t) {
   t->rows = kb == 4 ? 4 : 2;
-  t->waves = kb == 4 ? 4 : 8;
+  t->waves = kb == 4 ? 8 : 16;
   t->kblock = kb;
+  int occ = 0, zc = 0, got = 0;
+  long long tiles = 0;
+  if (h3d_pass_f64_geometry(g, kb, t->rows, t->waves, rhs, &occ, &tiles) == H3D_OK &&
+      h3d_round_zchunks(planes, kb, tiles, occ, &zc, 1, &got) == H3D_OK && got == 1) {
+    t->zchunk = zc;
+    return;
+  }
   const int OX = 64 - 2 * kb, OY = t->rows * t->waves - 2 * kb;
-  const long long tiles = (long long)((g->nx - 2 + OX - 1) / OX) *
-                          ((g->ny - 2 + OY - 1) / OY);
-  int zc = planes;
+  tiles = (long long)((g->nx - 2 + OX - 1) / OX) * ((g->ny - 2 + OY - 1) / OY);
+  zc = planes;
   while (zc > 8 * kb && tiles * ((planes + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
   t->zchunk = zc;
 }
@@ -229,12 +265,30 @@ extern "C" int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs
   if (nsteps == 1)
     return h3d_step_f64(u_out, u_in, rhs, kappa, g, factor, k_start, k_end, tune, stream);
   if (k_end < k_start) return H3D_OK;  // empty range: nothing to do
-  h3d_tune t;
-  default_tune_pass64(g, nsteps, k_end - k_start + 1, &t);
+  h3d_tune t{0, 0, 0, nsteps};
   if (tune) {
-    if (tune->rows) t.rows = tune->rows;
-    if (tune->waves) t.waves = tune->waves;
-    if (tune->zchunk) t.zchunk = tune->zchunk;
+    t.rows = tune->rows;
+    t.waves = tune->waves;
+    t.zchunk = tune->zchunk;
+  }
+  if (!t.rows || !t.waves || !t.zchunk) {
+    // the caller's non-zero fields, the rest from the default shape (a z
+    // chunk from the default rule for the shape in use)
+    h3d_tune d;
+    default_tune_pass64(g, nsteps, k_end - k_start + 1, rhs != nullptr, &d);
+    if (!t.zchunk && (t.rows || t.waves)) {
+      h3d_tune e{t.rows ? t.rows : d.rows, t.waves ? t.waves : d.waves, 0, nsteps};
+      int occ = 0, zc = 0, got = 0;
+      long long tiles = 0;
+      if (h3d_pass_f64_geometry(g, nsteps, e.rows, e.waves, rhs != nullptr, &occ, &tiles) ==
+              H3D_OK &&
+          h3d_round_zchunks(k_end - k_start + 1, nsteps, tiles, occ, &zc, 1, &got) == H3D_OK &&
+          got == 1)
+        d.zchunk = zc;
+    }
+    if (!t.rows) t.rows = d.rows;
+    if (!t.waves) t.waves = d.waves;
+    if (!t.zchunk) t.zchunk = d.zchunk;
   }
   if (t.zchunk < 1) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);

@@ -111,6 +111,9 @@ __device__ __forceinline__ float h3d_from_right(float v, float edge) {
 // Library-internal entry points shared between translation units.
 int h3d_default_tune_f64(const h3d_grid *g, int k_start, int k_end,
                          h3d_tune *t);
+// FP64 pass shape geometry: resident workgroups per CU, tiles per z chunk.
+int h3d_pass_f64_geometry(const h3d_grid *g, int kblock, int rows, int waves, int rhs,
+                          int *wg_per_cu, long long *tiles);
 
 // Passes captured in one hipGraph by the run functions (even, so that a
 // replay keeps the ping-pong parity); longer runs replay it.

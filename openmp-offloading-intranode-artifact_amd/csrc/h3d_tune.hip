@@ -162,11 +162,22 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       const std::vector<int> rws = kbk == 1 ? std::vector<int>{1, 2, 4} : std::vector<int>{2, 4};
       const std::vector<int> wvs =
           kbk == 1 ? std::vector<int>{2, 4, 8} : std::vector<int>{4, 8, 16};
-      const std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
-                                            : std::vector<int>{32, 64, 128};
       for (int rows : rws)
-        for (int waves : wvs)
+        for (int waves : wvs) {
+          // fused passes: 32..128-plane chunks and the chunks that fill
+          // whole rounds of resident workgroups (h3d_round_zchunks)
+          std::vector<int> zcs = kbk == 1 ? std::vector<int>{planes, 256, 128, 64, 32}
+                                          : std::vector<int>{32, 64, 128};
+          if (kbk > 1) {
+            int occ = 0, z[3], nz3 = 0;
+            long long tl = 0;
+            if (h3d_pass_f64_geometry(g, kbk, rows, waves, mode, &occ, &tl) != H3D_OK) continue;
+            if (h3d_round_zchunks(planes, kbk, tl, occ, z, 3, &nz3) != H3D_OK) nz3 = 0;
+            for (int i = 0; i < nz3; ++i)
+              if (std::find(zcs.begin(), zcs.end(), z[i]) == zcs.end()) zcs.push_back(z[i]);
+          }
           for (int zc : zcs) {
+            if (rc != H3D_OK) break;
             if (zc > planes && zc != zcs.front()) continue;
             h3d_tune t{rows, waves, zc, kbk};
             float ms;
@@ -174,6 +185,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
             if (r != H3D_OK && r != H3D_EARG && r != H3D_ELIMIT) { rc = r; break; }
             if (r == H3D_OK) cand.push_back({ms / kbk, t});
           }
+        }
     }
   } else if (rc == H3D_OK) {
     const int planes = g->nz - 2;
