@@ -203,6 +203,17 @@ __device__ __forceinline__ float lane_right(float v) {  // lane l <- l+1
 __device__ __forceinline__ float lane_left(float v) {  // lane l <- l-1
   return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
+// The same for the FP64 pass, one 32-bit half at a time.
+__device__ __forceinline__ double lane_right(double v) {  // lane l <- l+1
+  const int2 s = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_mov_dpp(s.x, 0x130, 0xf, 0xf, true),
+                                              __builtin_amdgcn_mov_dpp(s.y, 0x130, 0xf, 0xf, true)));
+}
+__device__ __forceinline__ double lane_left(double v) {  // lane l <- l-1
+  const int2 s = __builtin_bit_cast(int2, v);
+  return __builtin_bit_cast(double, make_int2(__builtin_amdgcn_mov_dpp(s.x, 0x138, 0xf, 0xf, true),
+                                              __builtin_amdgcn_mov_dpp(s.y, 0x138, 0xf, 0xf, true)));
+}
 
 __device__ __forceinline__ float ld_f32(__amdgpu_buffer_rsrc_t r, unsigned off) {
   return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
@@ -511,31 +522,29 @@ __device__ __forceinline__ void pass_f32_tile(
 // same operands (a flux computed for one cell is the opposite face's term
 // of its neighbour, bit for bit), u + factor*s or (u + rhs) + factor*s
 // without contraction, so KB fused steps give the bits of KB single steps.
-// With rhs, each lane keeps the rhs of its cells for the last KB planes in
-// a register ring (every level adds it; a cell reads only its own).
 //
-// kappa itself, not the face sums, sits in an LDS ring of KB+1 planes:
-// every sum k_a + k_b is formed where it is used, from the same two
-// operands, so it has the bits a stored sum would have.  Three sums per
-// cell and plane would take 3*KB planes of LDS; the kappa ring takes KB+1,
-// which lets two 8-wave workgroups of 32 rows share a CU at KB = 2 and fits
-// 32-row tiles at KB = 3 and 4.
+// The kappa face sums of a plane (the x, y and z faces above each cell
+// and the y face below a wave's first row) are formed once, when the
+// plane enters level 1, from the operands the single step adds, and kept
+// in a register ring of KB planes that levels 2..KB read: a sum formed
+// once has the bits of the same sum formed at every level.  kappa passes
+// through LDS once: plane it is written at iteration it and read at
+// iteration it+1 (a wave's own rows and the edge rows of the waves next
+// to it), so two LDS slots suffice.  With rhs, each lane keeps the rhs of
+// its cells for the last KB planes in a second register ring (every level
+// adds it; a cell reads only its own).  The march is unrolled by
+// lcm(2, KB) iterations, so every ring slot, level slot and LDS slot is a
+// compile-time index.
 
 // LDS of one workgroup, in doubles (rhs needs none):
-//   kr : [KB+1 slots][TY rows][64]        kappa of planes it-KB .. it
+//   kl : [2 slots][TY rows][64]           kappa of planes it-1 and it
 //   ue : [2][KB][W][2][64]  edge rows of every level, double buffered
 __host__ __device__ constexpr int pass64_lds_doubles(int KB, int RY, int W) {
-  return (KB + 1) * (W * RY) * 64 + 2 * KB * W * 2 * 64;
+  return 2 * (W * RY) * 64 + 2 * KB * W * 2 * 64;
 }
 
-// Planes loaded ahead and prefetch slots of the FP64 pass (as
-// pass_prefetch / pass_slots for the FP32 pass).
-__host__ __device__ constexpr int pass64_prefetch(int KB, int RY, int W, bool RHS) {
-  return 1;
-}
-__host__ __device__ constexpr int pass64_slots(int KB, int RY, int W, bool RHS) {
-  return 2;
-}
+// Iterations per unrolled group of the FP64 march: lcm(2, KB).
+__host__ __device__ constexpr int pass64_unroll(int KB) { return KB % 2 == 0 ? KB : 2 * KB; }
 
 template <int KB, int RY, int W, bool RHS>
 struct Pass64 {
@@ -544,15 +553,19 @@ struct Pass64 {
   static constexpr unsigned OUT_OF_RANGE = 0x80000000u;
   static constexpr int TY = W * RY;
   static constexpr int SLOT = TY * 64;  // doubles per kappa slot
+  static constexpr int U = pass64_unroll(KB);
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
-  static constexpr int PF = pass64_prefetch(KB, RY, W, RHS);  // planes ahead
-  static constexpr int NS = pass64_slots(KB, RY, W, RHS);     // prefetch slots
-  double pu[NS][RY];      // prefetched u rows (slot = iteration mod NS)
-  double pk[NS][RY];      // prefetched kappa rows
-  double pr[NS][RY];      // prefetched rhs rows (RHS)
-  double RH[KB][RY];      // rhs of planes it-1 .. it-KB (RHS; after the shift)
+  double pu[2][RY];       // prefetched u rows (slot = iteration parity)
+  double pk[2][RY];       // prefetched kappa rows
+  double pr[2][RY];       // prefetched rhs rows (RHS)
+  // rings over the last KB planes: plane q in slot (q - it0) mod KB
+  double CX[KB][RY];      // k(i+1) + k(i): x face above each cell
+  double CY[KB][RY];      // k(j+1) + k(j): y face above
+  double CZ[KB][RY];      // k(q) + k(q+1): z face above
+  double CB[KB];          // k(j) + k(j-1): y face below the wave's first row
+  double RH[KB][RY];      // rhs (RHS)
 
   // ---- geometry ----
   const double *uin;
@@ -567,17 +580,16 @@ struct Pass64 {
   bool lane_int;
   bool row_int[RY];
   int zb, ze;
-  double *kr;        // this lane, slot 0, this wave's row 0
+  double *kl;        // this lane, slot 0, this wave's row 0
   const double *kb;  // this lane, slot 0, the row below this wave's rows
   const double *ka;  // this lane, slot 0, the row above this wave's rows
   double *ue;        // this lane, (phase 0, level 0, wave 0, lo)
 
-  __device__ __forceinline__ int eslot(int p) const {  // p mod KB+1
-    const int s = p % (KB + 1);
-    return s < 0 ? s + KB + 1 : s;
-  }
   __device__ __forceinline__ static constexpr int ue_at(int ph, int l, int wave, int hi) {
     return (((ph * KB + l) * W + wave) * 2 + hi) * 64;
+  }
+  __device__ __forceinline__ static constexpr int ring(int ph) {  // ph mod KB, ph > -KB
+    return (ph + KB) % KB;
   }
 
   __device__ __forceinline__ void load_plane(int p, int ph) {
@@ -594,51 +606,57 @@ struct Pass64 {
     }
   }
 
-  // One march iteration at phase PH (iteration mod NS): the level slots
-  // alternate with P = PH mod 2, the prefetch slots cycle with PH.
+  // One march iteration, the PH-th of an unrolled group (it - it0 = PH
+  // mod U): level slots and LDS slots alternate with P = PH mod 2, the
+  // prefetch slots likewise, the rings cycle with PH mod KB.
   template <int PH>
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
     constexpr int P = PH & 1;
-    // 1. kappa of plane it into its ring slot (the slot of it-KB-1, which
-    //    no level reads any more).  Other waves read these rows from the
-    //    next iteration on, after the barrier; this lane's own rows of
-    //    plane it are taken from registers below.
-    {
-      double *ks = kr + eslot(it) * SLOT;
+    // 1. kappa of plane it into LDS slot P.  The slot held plane it-2,
+    //    last read at iteration it-1, before its closing barrier.
 #pragma unroll
-      for (int r = 0; r < RY; ++r) ks[r * 64] = pk[PH][r];
+    for (int r = 0; r < RY; ++r) kl[P * SLOT + r * 64] = pk[P][r];
+    // 2. Face sums of plane it-1 (level 1's plane) into ring slot PH-1:
+    //    its kappa rows, and the edge rows of the waves next to this one,
+    //    from LDS slot 1-P (written at iteration it-1); kappa of plane it
+    //    from registers.
+    {
+      constexpr int c1 = ring(PH - 1);
+      double kq[RY];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) kq[r] = kl[(1 - P) * SLOT + r * 64];
+      const double k_below = kb[(1 - P) * SLOT];
+      const double k_above = ka[(1 - P) * SLOT];
+#pragma unroll
+      for (int r = 0; r < RY; ++r) {
+        CX[c1][r] = lane_right(kq[r]) + kq[r];
+        CY[c1][r] = ((r < RY - 1) ? kq[r + 1] : k_above) + kq[r];
+        CZ[c1][r] = kq[r] + pk[P][r];
+      }
+      CB[c1] = kq[0] + k_below;
     }
     double rn[RY];
 #pragma unroll
-    for (int r = 0; r < RY; ++r) rn[r] = RHS ? pr[PH][r] : 0.0;  // rhs of plane it
-    // 2. Level 0: the loaded plane becomes the newest slot of level 0.
-    double kq1[RY];  // kappa of plane q+1 (this lane's rows)
+    for (int r = 0; r < RY; ++r) rn[r] = RHS ? pr[P][r] : 0.0;  // rhs of plane it
+    // 3. Level 0: the loaded plane becomes the newest slot of level 0.
 #pragma unroll
-    for (int r = 0; r < RY; ++r) {
-      A[0][1 - P][r] = pu[PH][r];
-      kq1[r] = pk[PH][r];
-    }
-    // 3. Prefetch the plane PF ahead (clamped, so always a valid address).
-    load_plane(it + PF, (PH + PF) % NS);
+    for (int r = 0; r < RY; ++r) A[0][1 - P][r] = pu[P][r];
+    // 4. Prefetch plane it+1 (clamped, so always a valid address).
+    load_plane(it + 1, 1 - P);
 
-    // 4. Levels 1..KB at planes it-1 .. it-KB.
+    // 5. Levels 1..KB at planes it-1 .. it-KB.
     const int wb = w > 0 ? w - 1 : 0;
     const int wa = w < W - 1 ? w + 1 : W - 1;
     const unsigned b8 = (unsigned)((unsigned long long)sz * 8ull);
 #pragma unroll
     for (int l = 1; l <= KB; ++l) {
+      const int c = ring(PH - l);  // ring slot of plane q
       const int q = it - l;
       const bool q_int = q >= 1 && q <= nz - 2;
       const bool q_out = (l == KB) && q >= zb && q <= ze;
       const double ub = ue[ue_at(1 - P, l - 1, wb, 1)];  // row below (mid plane)
       const double ua = ue[ue_at(1 - P, l - 1, wa, 0)];  // row above
-      const int so = eslot(q) * SLOT;
-      double kc[RY];
-#pragma unroll
-      for (int r = 0; r < RY; ++r) kc[r] = kr[so + r * 64];
-      const double k_below = kb[so];
-      const double k_above = ka[so];
       const long long qplane = (long long)(q_out ? q : zb) * sz;
       const auto ro = plane_rsrc(uout + qplane, b8);
       double fy_prev = 0.0;
@@ -646,21 +664,17 @@ struct Pass64 {
       for (int r = 0; r < RY; ++r) {
         const double u = A[l - 1][P][r];
         const double un = A[l - 1][1 - P][r];
-        const double k = kc[r];
-        const double ccx = h3d_from_right(k, 0.0) + k;
-        const double ccy = ((r < RY - 1) ? kc[r + 1] : k_above) + k;
-        const double ccz = k + kq1[r];
-        const double fxp = ccx * (h3d_from_right(u, 0.0) - u);
-        const double fxm = h3d_from_left(fxp, 0.0);
+        const double fxp = CX[c][r] * (lane_right(u) - u);
+        const double fxm = lane_left(fxp);
         const double ujp = (r < RY - 1) ? A[l - 1][P][r + 1] : ua;
-        const double fyp = ccy * (ujp - u);
-        const double fym = (r > 0) ? fy_prev : (k + k_below) * (u - ub);
+        const double fyp = CY[c][r] * (ujp - u);
+        const double fym = (r > 0) ? fy_prev : CB[c] * (u - ub);
         fy_prev = fyp;
-        const double fzp = ccz * (un - u);
+        const double fzp = CZ[c][r] * (un - u);
         const double fzm = FZ[l][r];
         FZ[l][r] = fzp;
         const double s = ((((fxp - fxm) + fyp) - fym) + fzp) - fzm;
-        const double res = RHS ? (u + RH[l - 1][r]) + factor * s : u + factor * s;
+        const double res = RHS ? (u + RH[c][r]) + factor * s : u + factor * s;
         if (l < KB) {
           // Dirichlet cells keep their value
           const bool inside = lane_int && row_int[r] && q_int;
@@ -672,25 +686,27 @@ struct Pass64 {
           st_f64(ro, q_out ? s8[r] : OUT_OF_RANGE, res);
         }
       }
-#pragma unroll
-      for (int r = 0; r < RY; ++r) kq1[r] = kc[r];
     }
-    // 5. rhs ring: plane it joins, plane it-KB leaves
+    // 6. rhs ring: plane it takes the slot of plane it-KB, which level KB
+    //    has just read
     if (RHS) {
 #pragma unroll
-      for (int l = KB - 1; l > 0; --l)
-#pragma unroll
-        for (int r = 0; r < RY; ++r) RH[l][r] = RH[l - 1][r];
-#pragma unroll
-      for (int r = 0; r < RY; ++r) RH[0][r] = rn[r];
+      for (int r = 0; r < RY; ++r) RH[ring(PH)][r] = rn[r];
     }
-    // 6. This iteration's newest edge rows of levels 0..KB-1.
+    // 7. This iteration's newest edge rows of levels 0..KB-1.
 #pragma unroll
     for (int l = 0; l < KB; ++l) {
       ue[ue_at(P, l, w, 0)] = A[l][1 - P][0];
       ue[ue_at(P, l, w, 1)] = A[l][1 - P][RY - 1];
     }
     __syncthreads();
+  }
+
+  // Iterations PH .. U-1 of a group starting at iteration it.
+  template <int PH>
+  __device__ __forceinline__ void group(int it) {
+    iter<PH>(it + PH);
+    if constexpr (PH + 1 < U) group<PH + 1>(it);
   }
 };
 
@@ -743,39 +759,35 @@ __device__ __forceinline__ void pass_f64_tile(
   // tile's own edge rows (row 0 of wave 0, the last row of wave W-1) take
   // their own row instead; their cells are off the output region at every
   // level above 0, so the value never reaches a stored result.
-  st.kr = lds + st.w * RY * 64 + lane;
+  st.kl = lds + st.w * RY * 64 + lane;
   st.kb = lds + (st.w > 0 ? st.w * RY - 1 : 0) * 64 + lane;
   st.ka = lds + (st.w < W - 1 ? (st.w + 1) * RY : S::TY - 1) * 64 + lane;
-  st.ue = lds + (KB + 1) * S::SLOT + lane;
+  st.ue = lds + 2 * S::SLOT + lane;
 
 #pragma unroll
   for (int l = 0; l <= KB; ++l)
 #pragma unroll
     for (int r = 0; r < RY; ++r) st.FZ[l][r] = 0.0;
 #pragma unroll
-  for (int r = 0; r < RY; ++r) {
+  for (int l = 0; l < KB; ++l) {
+    st.CB[l] = 0.0;
 #pragma unroll
-    for (int l = 0; l < KB; ++l) st.A[l][0][r] = st.A[l][1][r] = st.RH[l][r] = 0.0;
+    for (int r = 0; r < RY; ++r) {
+      st.A[l][0][r] = st.A[l][1][r] = st.RH[l][r] = 0.0;
+      st.CX[l][r] = st.CY[l][r] = st.CZ[l][r] = 0.0;
+    }
   }
   // All LDS starts defined (warm-up iterations read slots not yet written).
   for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W); x += W * 64) lds[x] = 0.0;
   __syncthreads();
 
-  // Iterations it0 .. cover zb-KB .. ze+KB in groups of NS; trailing
+  // Iterations it0 .. cover zb-KB .. ze+KB in groups of U; trailing
   // iterations of the last group write nothing (their planes are > ze).
   const int it0 = st.zb - KB;
-  constexpr int NS = S::NS;
-  const int groups = (st.ze + KB - it0 + NS) / NS;
-#pragma unroll
-  for (int f = 0; f < S::PF; ++f) st.load_plane(it0 + f, f);
-  for (int p = 0; p < groups; ++p) {
-    st.template iter<0>(it0 + NS * p);
-    st.template iter<1>(it0 + NS * p + 1);
-    if constexpr (NS == 4) {
-      st.template iter<2>(it0 + NS * p + 2);
-      st.template iter<3>(it0 + NS * p + 3);
-    }
-  }
+  constexpr int U = S::U;
+  const int groups = (st.ze + KB - it0 + U) / U;
+  st.load_plane(it0, 0);
+  for (int p = 0; p < groups; ++p) st.template group<0>(it0 + U * p);
 }
 
 // One tile (tx, ty, tz) of a single-step FP32 pass (KB = 1), built like the

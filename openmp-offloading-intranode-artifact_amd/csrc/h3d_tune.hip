@@ -67,8 +67,19 @@ int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, in
   H3D_TRY(hipGetDevice(&dev));
   H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const long long slots = (long long)(wg_per_cu > 0 ? wg_per_cu : 1) * (cus > 0 ? cus : 1);
+  // A workgroup marches z + 2*kblock planes plus a start-up worth about
+  // 8 more (its LDS clear and first loads).  The launch costs its total
+  // march over all slots, but no less than 0.85 of the makespan of whole
+  // rounds: a round with fewer workgroups than slots runs them faster
+  // (they share the bandwidth), yet not in proportion.  Measured on MI355X
+  // at 512^3 (3-step FP64 pass, 180 tiles): z chunks of 128, 103, 86 and
+  // 73 planes run in the ratios 1 : 1.02 : 1.06 : 1.07 this predicts, and
+  // 2 chunks (1.4 rounds) far slower.  Chunks are at most 256 planes: a
+  // whole 510-plane march per workgroup ran at half the rate of 128-plane
+  // chunks there, beyond what the model accounts for.
   struct C {
-    long long cost, rounds;
+    double cost;
+    long long rounds;
     int z;
   };
   std::vector<C> v;
@@ -78,9 +89,12 @@ int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, in
     if (z == last) continue;
     last = z;
     if (c > 1 && z < 4 * kblock) break;  // warm-up would exceed half the work
+    if (z > 256) continue;
     const long long wgs = tiles * ((planes + z - 1) / z);
     const long long rounds = (wgs + slots - 1) / slots;
-    v.push_back({rounds * (long long)(z + 2 * kblock), rounds, z});
+    const double march = (double)(z + 2 * kblock + 8);
+    const double cost = std::max((double)wgs * march / (double)slots, 0.85 * rounds * march);
+    v.push_back({cost, rounds, z});
   }
   // cheapest first; equal cost: fewer rounds (neighbouring tiles march
   // together in one round, so their shared halo lines meet in L2)
@@ -195,7 +209,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       // single-step passes use the wave-independent tiles (rows 1-4, any
       // wave count, long z marches); fused passes the LDS-shared tiles in
       // every shape the depth offers, each with the z chunks that fill
-      // whole rounds of resident workgroups (h3d_round_zchunks) and 64
+      // whole rounds of resident workgroups (h3d_round_zchunks), 64 and 128
       std::vector<std::pair<int, int>> shapes;
       if (kbk == 1) {
         for (int rows : {1, 2, 4})
@@ -217,7 +231,9 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
           int z[3], nz3 = 0;
           if (h3d_round_zchunks(planes, kbk, tl, occ, z, 3, &nz3) != H3D_OK) nz3 = 0;
           for (int i = 0; i < nz3; ++i) zcs.push_back(z[i]);
-          if (std::find(zcs.begin(), zcs.end(), 64) == zcs.end() && planes > 64) zcs.push_back(64);
+          for (int zf : {64, 128})
+            if (std::find(zcs.begin(), zcs.end(), zf) == zcs.end() && planes > zf)
+              zcs.push_back(zf);
         }
         for (int zc : zcs) {
           if (zc > planes && zc != zcs.front()) continue;
