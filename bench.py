@@ -436,7 +436,8 @@ def leg_cfg4(bw, stream):
     kernel run on the same (rounded / dequantised) inputs:
       - compensated and temporally blocked at the autotuner's depth, kappa
         in FP32 and in BF16 with a per-field scale, residual in registers
-        and in an FP16 field;
+        and in an FP16 field (that one also at depth 3 when the tuner picks
+        another);
       - the temporal-blocking depths 1, 2, 4 and 8 (FP32 kappa, residual in
         registers), each on its own tuned shape.
     Every run starts from a device-to-device copy of the initial field."""
@@ -530,6 +531,11 @@ def leg_cfg4(bw, stream):
     out["f32_kappa_carry"] = leg("f32_kappa_carry", fk32, fk32.ptr, None, f32f, 1, 0, ref, 4.0)
     out["f32_kappa_fp16_residual"] = leg("f32_kappa_fp16_residual", fk32, fk32.ptr, None,
                                          f32f, 2, 0, ref, 4.0)
+    if out["f32_kappa_fp16_residual"]["kblock"] != 3:
+        # the FP16-residual pass at depth 3 as well: the tuner's deeper pass
+        # can be faster per step while moving its fewer bytes at a lower rate
+        out["f32_kappa_fp16_residual_k3"] = leg("f32_kappa_fp16_residual_k3", fk32, fk32.ptr,
+                                                None, f32f, 2, 3, ref, 4.0)
     # the temporal-blocking depths of the configuration, each on its own
     # tuned shape (register carry; at k = 1 the carry spans one step)
     out["depth_sweep"] = {}

@@ -54,8 +54,18 @@
 
 namespace {
 
+// Waves per SIMD the compiler must leave room for: 4 (two 8-wave
+// workgroups per CU, 128 VGPRs) for the 4-step, 32-row tile with FP32
+// kappa and the residual in registers, which otherwise takes 129 VGPRs and
+// one workgroup per CU; with the bound it fits without spilling and ran
+// 1020 against 704 GLUP/s at 1024^3 (the other shapes either fit already
+// or would spill).
+__host__ __device__ constexpr int pass_waves_per_eu(int KB, int RY, int W, int MODE, bool BF16) {
+  return (KB == 4 && RY == 4 && W == 8 && MODE == 1 && !BF16) ? 4 : 1;
+}
+
 template <int KB, int RY, int W, int MODE, bool BF16>
-__global__ void __launch_bounds__(W * 64)
+__global__ void __launch_bounds__(W * 64, pass_waves_per_eu(KB, RY, W, MODE, BF16))
 h3d_pass_f32_kernel(float *__restrict__ uout, const float *__restrict__ uin,
                     const float *__restrict__ kf,
                     const unsigned short *__restrict__ kb16,
