@@ -149,6 +149,15 @@ def note_launch(label, key, alg_bytes):
     log(f"launch {label} key={key} alg_bytes={int(alg_bytes)}")
 
 
+def fp64_passes(g, steps, tune):
+    """Launches h3d_run_f64 makes for `steps` with this tune on grid g
+    without rhs (a remainder may ride on deeper passes)."""
+    n, d = h3d.C.c_int(), h3d.C.c_int()
+    h3d.call("h3d_run_f64_passes", h3d.C.byref(g), 0, steps, h3d.C.byref(tune),
+             h3d.C.byref(n), h3d.C.byref(d))
+    return n.value
+
+
 def config_steps(own):
     return ARGS.config_steps if ARGS and ARGS.config_steps > 0 else own
 
@@ -728,7 +737,7 @@ def main():
             return dev
 
         wall, dev_ms, walls = repeat_timed(prepare, run)
-        launches = -(-a.steps // max(1, tune.kblock))
+        launches = fp64_passes(g, a.steps, tune)
         return cells * a.steps / wall / 1e9, wall, dev_ms / 1e3 / launches, state["out"], walls
 
     def tuned64(fixed_kblock):

@@ -54,7 +54,7 @@ typedef struct h3d_tune {
   int rows;   /* y rows per lane held in registers                   */
   int waves;  /* wavefronts per workgroup, stacked in y               */
   int zchunk; /* planes marched by one workgroup                      */
-  int kblock; /* time steps fused per HBM pass: FP32 1..6, FP64 1..4  */
+  int kblock; /* time steps fused per HBM pass: FP32 1..8, FP64 1..4  */
 } h3d_tune;
 
 /* ---------------------------------------------------------------------- */
@@ -115,14 +115,27 @@ int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
 
 /* `steps` FP64 updates with the u_a/u_b ping-pong of the reference's time
  * loop (1-openmp.c:193-205).  With tune->kblock = K > 1, every K steps are
- * one h3d_pass_f64 (the remainder one shorter pass), and the buffers swap
- * once per pass.  graph != 0 records two steps (passes) in a hipGraph and
- * replays it.  *ms receives the device time of the loop measured with HIP
- * events on `stream`; *final_in_b is 1 if the result is in u_b. */
+ * one h3d_pass_f64 and the buffers swap once per pass.  A remainder of r
+ * steps is one shorter pass; without rhs and for K = 2 or 3, when the run
+ * has at least r passes and h3d_autotune has timed depths K and K+1 on
+ * this grid (it does so when it picks K), the last r passes are K+1 steps
+ * deep instead (the tuner's K+1 shape) if that costs less by those times,
+ * one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
+ * passes (up to 1024 of them; longer runs replay a graph of 1024 K-step
+ * passes) in a hipGraph and launches it.  *ms receives
+ * the device time of the loop measured with HIP events on `stream`;
+ * *final_in_b is 1 if the result is in u_b (an odd number of passes). */
 int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
                 const double *kappa, const h3d_grid *g, double factor,
                 int steps, const h3d_tune *tune, int graph, void *stream,
                 float *ms, int *final_in_b);
+
+/* The passes h3d_run_f64 launches for `steps` with this tune on grid g
+ * (has_rhs: whether rhs is passed) -> *passes, of which *deep_passes
+ * (may be NULL) are one step deeper than tune->kblock.  No reference
+ * counterpart. */
+int h3d_run_f64_passes(const h3d_grid *g, int has_rhs, int steps,
+                       const h3d_tune *tune, int *passes, int *deep_passes);
 
 /* Face pack/unpack: the whole XY plane k including its x/y ring, written
  * to/read from a dense [j][i] buffer of nx*ny values -- the order of the
@@ -240,27 +253,31 @@ int h3d_face_sum_max_f64(const double *kappa, const h3d_grid *g, double *out,
 /* ---------------------------------------------------------------------- */
 /* FP32 storage paths (no reference counterpart; the reference is FP64).   */
 
-#define H3D_F32_PLAIN 0 /* u' = u + d                                      */
+#define H3D_F32_PLAIN 0 /* u' = fma(factor, S, u)                          */
 #define H3D_F32_CARRY 1 /* compensated; residual carried in registers
                            across the fused steps of one pass             */
 #define H3D_F32_COMP16 2 /* compensated; residual also kept between passes
                            in an FP16 field scaled by the cell's ulp      */
 
-/* One HBM pass of `nsteps` (1..6) fused time steps over the interior of
+/* One HBM pass of `nsteps` (1..8) fused time steps over the interior of
  * a grid stored in FP32.  kappa is exactly one of kappa_f32 / kappa_bf16
  * (BF16 bits of kappa/scale; pass factor*scale as `factor`).  comp_in /
  * comp_out (FP16 bits) are used in mode H3D_F32_COMP16 only.  Cell update:
- * S in the FP64 kernel's order in FP32, d = factor*S, then Kahan's
- * compensated sum: y = d + e, u' = u + y, e' = y - (u' - u).  u_out's ring must hold the Dirichlet values. */
+ * S in the FP64 kernel's order in FP32, then Kahan's compensated sum with
+ * the increment joining the residual in one rounding: y = fma(factor, S,
+ * e), u' = u + y, e' = y - (u' - u).  u_out's ring must hold the Dirichlet
+ * values. */
 int h3d_pass_f32(float *u_out, const float *u_in, const float *kappa_f32,
                  const uint16_t *kappa_bf16, uint16_t *comp_out,
                  const uint16_t *comp_in, const h3d_grid *g, float factor,
                  int nsteps, int mode, const h3d_tune *tune, void *stream);
 
 /* `steps` FP32 updates as passes of tune->kblock steps (remainder last),
- * ping-ponging u_a/u_b (and c_a/c_b in COMP16 mode).  graph != 0 replays a
- * captured pair of passes.  *ms: loop time by HIP events on `stream`;
- * *final_in_b: 1 if the result (and residual field) ended in u_b (c_b). */
+ * ping-ponging u_a/u_b (and c_a/c_b in COMP16 mode).  graph != 0 captures
+ * the run's passes (up to 1024; longer runs replay a graph of 1024 passes)
+ * in a hipGraph and launches it.  *ms: loop time by HIP events on
+ * `stream`; *final_in_b: 1 if the result (and residual field) ended in
+ * u_b (c_b). */
 int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
                 const uint16_t *kappa_bf16, uint16_t *c_a, uint16_t *c_b,
                 const h3d_grid *g, float factor, int steps, int mode,
@@ -273,9 +290,10 @@ int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
  * is not used); results are cached per
  * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step
  * or pass (mode 0: no rhs, mode 1: with rhs, which the shapes' LDS must
- * hold; kblock chosen too, 1..3, unless fixed_kblock > 0), 1 = FP32 pass
- * (kblock chosen too, 1..4, unless fixed_kblock > 0).  The figure of merit
- * is time per step.  The caller's buffers are not modified. */
+ * hold; kblock chosen too, 1..3, unless fixed_kblock 1..4 is given), 1 =
+ * FP32 pass (kblock chosen too, 1..4, unless fixed_kblock 1..8 is given).
+ * The figure of merit is time per step.  The caller's buffers are not
+ * modified. */
 int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
                  int kappa_bf16, const h3d_grid *g, int mode,
                  int fixed_kblock, void *stream, h3d_tune *out);

@@ -356,6 +356,42 @@ extern "C" int h3d_heat_update_boundary_top(double *u_new, const double *u_old,
 // ---------------------------------------------------------------------------
 // Time loop with ping-pong buffers, optionally replayed from a hipGraph.
 
+// The passes of a run of `steps` at pass depth kb: `full` passes of kb
+// steps, the last `deep` of them one step deeper (shape *tdeep), then one
+// pass of `rem` steps when rem > 0.  A remainder of r steps is one shorter
+// pass, or, without rhs at depths 2 and 3, the last r passes one step
+// deeper when the autotuner has timed both depths on this grid and r
+// deeper passes cost less than a pass: a pass moves the same bytes at any
+// depth and costs about as much, so the deeper passes win when a
+// (K+1)-step pass costs less than (1 + 1/r) K-step passes.
+static void run_plan_f64(const h3d_grid *g, bool has_rhs, int steps, int kb, int *full,
+                         int *deep, int *rem, h3d_tune *tdeep) {
+  *full = steps / kb;
+  *rem = steps % kb;
+  *deep = 0;
+  if (has_rhs || !(kb == 2 || kb == 3) || *rem == 0 || *full < *rem) return;
+  float t_kb = 0.f, t_deep = 0.f;
+  if (h3d_tuned_f64(g, 0, kb, nullptr, &t_kb) == H3D_OK &&
+      h3d_tuned_f64(g, 0, kb + 1, tdeep, &t_deep) == H3D_OK &&
+      (float)*rem * (t_deep - t_kb) < t_kb) {
+    *deep = *rem;
+    *rem = 0;
+  }
+}
+
+extern "C" int h3d_run_f64_passes(const h3d_grid *g, int has_rhs, int steps,
+                                  const h3d_tune *tune, int *passes, int *deep_passes) {
+  if (!h3d_grid_ok(g) || steps < 0 || !passes) return H3D_EARG;
+  const int kb = (tune && tune->kblock > 1) ? tune->kblock : 1;
+  if (kb > 4) return H3D_ELIMIT;
+  int full = 0, deep = 0, rem = 0;
+  h3d_tune tdeep{};
+  run_plan_f64(g, has_rhs != 0, steps, kb, &full, &deep, &rem, &tdeep);
+  *passes = full + (rem ? 1 : 0);
+  if (deep_passes) *deep_passes = deep;
+  return H3D_OK;
+}
+
 extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
                            const double *kappa, const h3d_grid *g,
                            double factor, int steps, const h3d_tune *tune,
@@ -364,31 +400,40 @@ extern "C" int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
   if (!u_a || !u_b || !kappa || !h3d_grid_ok(g) || steps < 0) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   if (graph && !s) return H3D_EARG;  // capture needs a created stream
-  // kblock > 1: passes of kblock steps each (h3d_pass_f64), the remainder
-  // as one shorter pass; every pass swaps the buffers once
   const int kb = (tune && tune->kblock > 1) ? tune->kblock : 1;
   if (kb > 4) return H3D_ELIMIT;
-  auto pass = [&](double *out, const double *in, int ns) {
-    return ns == 1 ? h3d_step_f64(out, in, rhs, kappa, g, factor, 1, g->nz - 2, tune, s)
-                   : h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, 1, g->nz - 2, tune, s);
-  };
-  const int full = steps / kb, rem = steps % kb;
+  int full = 0, deep = 0, rem = 0;
+  h3d_tune tdeep{};
+  run_plan_f64(g, rhs != nullptr, steps, kb, &full, &deep, &rem, &tdeep);
   const int npass = full + (rem ? 1 : 0);
+  auto pass = [&](double *out, const double *in, int ns) {
+    if (ns == 1) return h3d_step_f64(out, in, rhs, kappa, g, factor, 1, g->nz - 2, tune, s);
+    return h3d_pass_f64(out, in, rhs, kappa, g, factor, ns, 1, g->nz - 2,
+                        ns == kb + 1 ? &tdeep : tune, s);
+  };
   auto pass_p = [&](int p) {
     const bool even = (p % 2) == 0;
-    return pass(even ? u_b : u_a, even ? u_a : u_b, p < full ? kb : rem);
+    const int ns = p < full - deep ? kb : (p < full ? kb + 1 : rem);
+    return pass(even ? u_b : u_a, even ? u_a : u_b, ns);
   };
   h3d_run_res R;
   H3D_TRY(hipEventCreate(&R.e0));
   H3D_TRY(hipEventCreate(&R.e1));
   int rc = H3D_OK;
   // graph: the run's passes captured once and launched as one graph (runs
-  // of more than H3D_GRAPH_PASSES passes replay a graph of that many full
-  // passes, the rest launched directly)
+  // of more than H3D_GRAPH_PASSES passes replay a graph of that many
+  // kblock-step passes, the rest launched directly)
   int in_graph = 0, reps = 0;
   if (graph && npass >= 2) {
-    in_graph = npass <= H3D_GRAPH_PASSES ? npass : H3D_GRAPH_PASSES;
-    reps = npass <= H3D_GRAPH_PASSES ? 1 : full / H3D_GRAPH_PASSES;
+    if (npass <= H3D_GRAPH_PASSES) {
+      in_graph = npass;
+      reps = 1;
+    } else {
+      reps = (full - deep) / H3D_GRAPH_PASSES;
+      in_graph = reps ? H3D_GRAPH_PASSES : 0;
+    }
+  }
+  if (in_graph) {
     H3D_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int r = H3D_OK;
     for (int p = 0; r == H3D_OK && p < in_graph; ++p) r = pass_p(p);

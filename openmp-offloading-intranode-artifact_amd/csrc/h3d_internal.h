@@ -127,6 +127,11 @@ int h3d_pass_f32_geometry(const h3d_grid *g, int kblock, int rows, int waves, in
 int h3d_pass_f32_zchunks(const h3d_grid *g, int kblock, int rows, int waves, int mode,
                          int bf16, int *zc, int n, int *got);
 
+// The autotuner's shape and device time per pass for an FP64 problem on
+// grid g (mode 1: with rhs) at pass depth kblock, from a search at that
+// fixed depth or a free search that chose it; H3D_ELIMIT when it has none.
+int h3d_tuned_f64(const h3d_grid *g, int mode, int kblock, h3d_tune *t, float *ms_per_pass);
+
 // Up to n z-chunk lengths (best first) for a pass of `kblock` steps over
 // `planes` planes whose launch has `tiles` workgroups per z chunk and
 // `wg_per_cu` resident workgroups per CU; *got receives how many were

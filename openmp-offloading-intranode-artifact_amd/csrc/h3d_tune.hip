@@ -28,8 +28,14 @@ struct Key {
   }
 };
 
+// The winner per problem key, with its device time per pass.
+struct Entry {
+  Key key;
+  h3d_tune t;
+  float ms_per_pass;
+};
 std::mutex cache_mu;
-std::vector<std::pair<Key, h3d_tune>> cache;
+std::vector<Entry> cache;
 
 // Best of three samples of `nl` launches (ms per launch), after one
 // warm-up launch; launch(j) is the j-th launch of a sample.
@@ -121,7 +127,7 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   {
     std::lock_guard<std::mutex> lk(cache_mu);
     for (auto &e : cache)
-      if (e.first == key) { *out = e.second; return H3D_OK; }
+      if (e.key == key) { *out = e.t; return H3D_OK; }
   }
   hipStream_t s = h3d_stream(stream);
   const size_t cells = h3d_grid_elems(g);
@@ -268,8 +274,33 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   if (best_ms >= 1e30f) return H3D_ELIMIT;  // no candidate ran
   {
     std::lock_guard<std::mutex> lk(cache_mu);
-    cache.push_back({key, best});
+    cache.push_back({key, best, best_ms * (float)best.kblock});
   }
   *out = best;
+  // FP64 without rhs at a free depth of 2 or 3: the next depth as well, so
+  // that h3d_run_f64 can weigh deeper passes against a short remainder
+  // pass on measured times (h3d_tuned_f64)
+  if (kind == 0 && mode == 0 && fixed_kblock == 0 && (best.kblock == 2 || best.kblock == 3)) {
+    h3d_tune deeper;
+    const int r2 = h3d_autotune(kind, u_a, u_b, kappa, kappa_bf16, g, mode, best.kblock + 1,
+                                stream, &deeper);
+    if (r2 != H3D_OK && r2 != H3D_ELIMIT) return r2;
+  }
   return H3D_OK;
+}
+
+int h3d_tuned_f64(const h3d_grid *g, int mode, int kblock, h3d_tune *t, float *ms_per_pass) {
+  if (!g) return H3D_EARG;
+  std::lock_guard<std::mutex> lk(cache_mu);
+  for (auto &e : cache) {
+    const Key &k = e.key;
+    if (k.kind != 0 || k.nx != g->nx || k.ny != g->ny || k.nz != g->nz || k.sy != g->sy ||
+        k.sz != g->sz || k.mode != mode || e.t.kblock != kblock)
+      continue;
+    if (k.fixed_kb != 0 && k.fixed_kb != kblock) continue;
+    if (t) *t = e.t;
+    if (ms_per_pass) *ms_per_pass = e.ms_per_pass;
+    return H3D_OK;
+  }
+  return H3D_ELIMIT;
 }

@@ -64,6 +64,7 @@ SIGNATURES = {
     "h3d_pass_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _I, _I, _TP, _P]),
     "h3d_run_f64": (_I, [_P, _P, _P, _P, _GP, _D, _I, _TP, _I, _P,
                          C.POINTER(C.c_float), C.POINTER(_I)]),
+    "h3d_run_f64_passes": (_I, [_GP, _I, _I, _TP, C.POINTER(_I), C.POINTER(_I)]),
     "h3d_slab_step_f64": (_I, [_P, _P, _P, _P, _GP, _D, _P, _P, _TP, _P]),
     "h3d_run_slab_f64": (_I, [_P, _P, _P, _P, _GP, _D, _P, _P, _I, _TP, _I, _P,
                               C.POINTER(C.c_float), C.POINTER(_I)]),

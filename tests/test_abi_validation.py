@@ -172,3 +172,23 @@ def test_ensemble_and_tuner_checks(h3d, L):
     assert L.h3d_autotune(0, u, v, k, 0, C.byref(g), 0, 5, None, C.byref(t)) == ELIMIT
     bad = grid(h3d, nx=2)
     assert L.h3d_autotune(0, u, v, k, 0, C.byref(bad), 0, 0, None, C.byref(t)) == EARG
+
+
+def test_run_plan_query(h3d, L):
+    """h3d_run_f64_passes: the launches of a run, on the CPU (it reads only
+    the autotuner's cache, empty for this grid)."""
+    g = grid(h3d, nx=23, ny=19, nz=17)
+    n, d = C.c_int(), C.c_int()
+    for kb, steps, want in ((3, 20, 7), (3, 21, 7), (2, 5, 3), (1, 9, 9), (4, 9, 3), (0, 4, 4)):
+        t = h3d.Tune(2, 8, 0, kb)
+        assert L.h3d_run_f64_passes(C.byref(g), 0, steps, C.byref(t), C.byref(n),
+                                    C.byref(d)) == 0
+        assert (n.value, d.value) == (want, 0)
+    assert L.h3d_run_f64_passes(C.byref(g), 1, 20, None, C.byref(n), None) == 0
+    assert n.value == 20
+    t = h3d.Tune(2, 8, 0, 5)
+    assert L.h3d_run_f64_passes(C.byref(g), 0, 20, C.byref(t), C.byref(n), None) == ELIMIT
+    assert L.h3d_run_f64_passes(C.byref(g), 0, -1, None, C.byref(n), None) == EARG
+    assert L.h3d_run_f64_passes(C.byref(g), 0, 4, None, None, None) == EARG
+    bad = grid(h3d, nx=2)
+    assert L.h3d_run_f64_passes(C.byref(bad), 0, 4, None, C.byref(n), None) == EARG

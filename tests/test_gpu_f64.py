@@ -274,17 +274,19 @@ PASS_CASES = [
 ]
 
 
+@pytest.mark.parametrize("extra", [0, 1], ids=["2k+1", "3k+k-1"])
 @pytest.mark.parametrize("case", PASS_CASES, ids=str)
-def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case):
-    """Temporally blocked FP64 passes (h3d_run_f64 with kblock > 1): two
-    full passes and a remainder, bit-identical to the oracle stepping the
-    reference's update one step at a time."""
+def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case, extra):
+    """Temporally blocked FP64 passes (h3d_run_f64 with kblock > 1): full
+    passes and a remainder of 1 or kblock-1 steps, bit-identical to the
+    oracle stepping the reference's update one step at a time, in the
+    buffer the pass count (h3d_run_f64_passes) implies."""
     shape, kb, rows, waves, zc, with_rhs = case
     nx, ny, nz = shape
     g, u, k = seeded(gpu, nx, ny, nz, kind=1)
     rhs = np.random.default_rng(3).uniform(-1e-4, 1e-4, g.elems) if with_rhs else None
     factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
-    steps = 2 * kb + 1
+    steps = 2 * kb + 1 if extra == 0 else 4 * kb - 1
     want = oracle_steps(oracle, g, u, k, factor, steps, rhs=rhs)
     da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
     dr = gpu.DeviceArray.from_host(rhs) if with_rhs else None
@@ -293,7 +295,11 @@ def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case):
     gpu.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr if dr else None, dk.ptr, gpu.C.byref(g),
              factor, steps, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
     got = (db if fb.value else da).download()
-    assert fb.value == 1  # three passes
+    npass = gpu.C.c_int()
+    gpu.call("h3d_run_f64_passes", gpu.C.byref(g), 1 if with_rhs else 0, steps, gpu.C.byref(t),
+             gpu.C.byref(npass), None)
+    assert npass.value in (steps // kb, -(-steps // kb))
+    assert fb.value == npass.value % 2
     assert np.array_equal(got, want)
 
 
@@ -400,3 +406,32 @@ def test_fused_pass_argument_errors(gpu):
     ms, fb = gpu.C.c_float(), gpu.C.c_int()
     assert L.h3d_run_f64(d1.ptr, d2.ptr, None, d2.ptr, gpu.C.byref(g), f, 5, gpu.C.byref(t), 0,
                          None, gpu.C.byref(ms), gpu.C.byref(fb)) == -2
+
+
+def test_tuned_run_with_deeper_remainder_passes(gpu, oracle):
+    """The autotuner's free FP64 search also times the next pass depth; a
+    run whose step count leaves a remainder may then carry it on deeper
+    passes (h3d_run_f64_passes says how many).  Either way the result is
+    the bits of single steps, in the buffer the pass count implies."""
+    C = gpu.C
+    nx, ny, nz = 134, 134, 134
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1)
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+    t = gpu.Tune()
+    gpu.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, C.byref(g), 0, 0, None, C.byref(t))
+    kb = max(1, t.kblock)
+    for steps in (2 * kb + 1, 3 * kb + kb - 1):
+        n, d = C.c_int(), C.c_int()
+        gpu.call("h3d_run_f64_passes", C.byref(g), 0, steps, C.byref(t), C.byref(n), C.byref(d))
+        full, rem = divmod(steps, kb)
+        assert n.value in (full, full + (1 if rem else 0))
+        assert d.value == (rem if n.value == full and rem else 0)
+        want = oracle_steps(oracle, g, u, k, factor, steps)
+        da.upload(u)
+        db.upload(u)
+        ms, fb = C.c_float(), C.c_int()
+        gpu.call("h3d_run_f64", da.ptr, db.ptr, None, dk.ptr, C.byref(g), factor, steps,
+                 C.byref(t), 0, None, C.byref(ms), C.byref(fb))
+        assert fb.value == n.value % 2
+        assert np.array_equal((db if fb.value else da).download(), want)
