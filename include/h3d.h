@@ -119,8 +119,8 @@ int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
  * steps is one shorter pass; without rhs and for K = 2 or 3, when the run
  * has at least r passes and h3d_autotune has timed depths K and K+1 on
  * this grid (it does so when it picks K), the last r passes are K+1 steps
- * deep instead (the tuner's K+1 shape) if that costs less by those times,
- * one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
+ * deep instead (the tuner's K+1 shape) if by those times that saves more
+ * than a quarter of a K-step pass, one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
  * passes (up to 1024 of them; longer runs replay a graph of 1024 K-step
  * passes) in a hipGraph and launches it.  *ms receives
  * the device time of the loop measured with HIP events on `stream`;

@@ -4,9 +4,10 @@
 //
 // Each member is cut into tiles (x segments of 64 lanes, y strips, z
 // chunks of `zchunk` planes: the z-march depth).  A work list holds one
-// int4 {member, tx, ty, tz} per tile, longest chunks first, and a fixed
-// grid of workgroups -- as many as are resident on the device at once --
-// walks it with a grid stride.  FP64 and FP32 members have a list and a
+// int4 {member, tx, ty, tz} per tile, longest chunks first, split into
+// one queue per XCD, and a fixed grid of workgroups -- as many as are
+// resident on the device at once -- takes tiles from the queues until all
+// are done (ensemble_step).  FP64 and FP32 members have a list and a
 // kernel each, launched one after the other, so each kernel runs at the
 // occupancy of its own tile:
 //  - FP64 members: 1024-thread workgroups of 32 rows (16 waves x 2 rows),
@@ -57,55 +58,97 @@ struct EMember {
   int precision;
 };
 
-// Tiles of members of precision PREC (0 FP64, 1 FP32) at pass depth KB;
-// parity: which buffer every member reads this launch.
+// LDS of the (depth, precision) kernel in bytes: the tile's, then one
+// int for the tile index the workgroup has taken from its queue (kept at
+// the end, so the tile's doubles stay 8-byte aligned).
 template <int KB, int PREC>
-__global__ void __launch_bounds__(ens_waves<PREC>() * 64)
-ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
-              int nwork, int parity, int zchunk) {
-  extern __shared__ float lds[];
-  for (int t = blockIdx.x; t < nwork; t += gridDim.x) {
-    const int4 d = work[t];
-    const EMember m = mem[d.x];
-    if constexpr (PREC == 0) {
-      const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
-      double *out = static_cast<double *>(parity ? m.ua : m.ub);
-      if constexpr (KB > 1) {
-        const int zb = 1 + d.w * zchunk;
-        pass_f64_tile<KB, ENS64_PRY, ENS64_WAVES, false>(
-            reinterpret_cast<double *>(lds), out, in, nullptr,
-            static_cast<const double *>(m.kap), m.nx, m.ny, m.nz, m.sy, m.sz, m.factor, zb,
-            min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
-      } else {
-        march_f64_tile<ENS64_ROWS, false>(out, in, nullptr,
-                                          static_cast<const double *>(m.kap),
-                                          m.nx, m.ny, m.sy, m.sz, m.factor, 1,
-                                          m.nz - 2, zchunk, 1, d.y, d.z, d.w);
-      }
-    } else {
-      const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
-      float *out = static_cast<float *>(parity ? m.ua : m.ub);
-      const unsigned short *cin = parity ? m.cb : m.ca;
-      unsigned short *cout = parity ? m.ca : m.cb;
-      const int zb = 1 + d.w * zchunk;
-      pass_f32_tile<KB, ENS32_RY, ENS32_WAVES, 2, false>(
-          lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
-          m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
-          min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
-    }
-    // the next tile's set-up rewrites LDS
-    __syncthreads();
-  }
-}
-
-template <int KB, int PREC>
-constexpr size_t ens_lds() {
+constexpr size_t ens_tile_lds() {
   if constexpr (PREC == 1)
     return sizeof(float) * (size_t)pass_lds_floats(KB, ENS32_RY, ENS32_WAVES);
   else
     return KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS64_PRY, ENS64_WAVES)
                   : 0;
 }
+template <int KB, int PREC>
+constexpr size_t ens_lds() { return ens_tile_lds<KB, PREC>() + 16; }
+
+// XCD queues of a work list: workgroup b runs on XCD b mod 8 and takes
+// its tiles from that XCD's queue first.
+constexpr int ENS_QUEUES = 8;
+
+// Tiles of members of precision PREC (0 FP64, 1 FP32) at pass depth KB;
+// parity: which buffer every member reads this launch.
+//
+// The list is split into ENS_QUEUES queues, work[qoff[x] .. qoff[x+1]),
+// one per XCD, each costliest tile first.  A workgroup takes the next tile
+// of its own XCD's queue (an atomic counter per queue) whenever it has
+// finished one, and once that queue is empty takes tiles from the other
+// queues, so every workgroup stays busy until the whole list is done
+// (list scheduling; a fixed grid-stride assignment left workgroups idle
+// behind slower neighbours).  Each queue holds runs of consecutive tiles
+// as long as its XCD has workgroups, so neighbouring tiles of a member
+// chunk march together under one L2 and share their halo lines there.
+// The last workgroup to finish zeroes the counters for the next launch,
+// so a launch needs no reset of its own (none inside a captured graph).
+template <int KB, int PREC>
+__global__ void __launch_bounds__(ens_waves<PREC>() * 64)
+ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
+              const int *__restrict__ qoff, int *qcnt, int parity, int zchunk) {
+  extern __shared__ float lds[];
+  int *grabbed = reinterpret_cast<int *>(reinterpret_cast<char *>(lds) + ens_tile_lds<KB, PREC>());
+  const int home = (int)(blockIdx.x % ENS_QUEUES);
+  for (int q = 0; q < ENS_QUEUES; ++q) {
+    const int x = (home + q) % ENS_QUEUES;
+    const int lo = qoff[x], n = qoff[x + 1] - lo;
+    for (;;) {
+      if (threadIdx.x == 0) *grabbed = atomicAdd(&qcnt[x], 1);
+      __syncthreads();
+      const int t = *grabbed;
+      __syncthreads();  // every thread has read it before it is rewritten
+      if (t >= n) break;  // workgroup-uniform
+      const int4 d = work[lo + t];
+      const EMember m = mem[d.x];
+      if constexpr (PREC == 0) {
+        const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
+        double *out = static_cast<double *>(parity ? m.ua : m.ub);
+        if constexpr (KB > 1) {
+          const int zb = 1 + d.w * zchunk;
+          pass_f64_tile<KB, ENS64_PRY, ENS64_WAVES, false>(
+              reinterpret_cast<double *>(lds), out, in, nullptr,
+              static_cast<const double *>(m.kap), m.nx, m.ny, m.nz, m.sy, m.sz, m.factor, zb,
+              min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
+        } else {
+          march_f64_tile<ENS64_ROWS, false>(out, in, nullptr,
+                                            static_cast<const double *>(m.kap),
+                                            m.nx, m.ny, m.sy, m.sz, m.factor, 1,
+                                            m.nz - 2, zchunk, 1, d.y, d.z, d.w);
+        }
+      } else {
+        const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
+        float *out = static_cast<float *>(parity ? m.ua : m.ub);
+        const unsigned short *cin = parity ? m.cb : m.ca;
+        unsigned short *cout = parity ? m.ca : m.cb;
+        const int zb = 1 + d.w * zchunk;
+        pass_f32_tile<KB, ENS32_RY, ENS32_WAVES, 2, false>(
+            lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
+            m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
+            min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
+      }
+      // the next tile's set-up rewrites LDS
+      __syncthreads();
+    }
+  }
+  // Every workgroup counts itself out; the last one (all others have
+  // finished taking tiles) zeroes the queue counters and the count.
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&qcnt[ENS_QUEUES], 1) == (int)gridDim.x - 1) {
+      for (int x = 0; x <= ENS_QUEUES; ++x) atomicExch(&qcnt[x], 0);
+      __threadfence();
+    }
+  }
+}
+
 
 }  // namespace
 
@@ -127,7 +170,12 @@ struct h3d_ensemble {
   // precision, with the persistent grid of each kernel.
   int4 *d_work;
   int work_cap;
-  int off[2][2], n[2][2], blocks[2][2];
+  int n[2][2], blocks[2][2];
+  // per (list, precision): the ENS_QUEUES + 1 queue offsets into d_work
+  // (d_qoff + (2 * list + precision) * 16) and the queue counters plus the
+  // finished-workgroup count (d_qcnt, same layout; zero between launches)
+  int *d_qoff;
+  int *d_qcnt;
   std::vector<unsigned short *> comp;  // owned FP16 residual fields
   // cached graph of `graph_steps` steps starting at buffer parity graph_p
   hipGraphExec_t graph = nullptr;
@@ -149,6 +197,8 @@ static void ensemble_free(h3d_ensemble *e) {
   drop_graph(e);
   if (e->d_mem) (void)hipFree(e->d_mem);
   if (e->d_work) (void)hipFree(e->d_work);
+  if (e->d_qoff) (void)hipFree(e->d_qoff);
+  if (e->d_qcnt) (void)hipFree(e->d_qcnt);
   for (auto *c : e->comp) (void)hipFree(c);
   delete e;
 }
@@ -212,26 +262,46 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   std::vector<int4> lists[2][2];
   for (int prec = 0; prec < 2; ++prec) {
     lists[0][prec] = list(kb, prec);
-    if (kb > 1) lists[1][prec] = list(1, prec);
+    lists[1][prec] = kb > 1 ? list(1, prec) : lists[0][prec];
   }
-  size_t total = 0;
-  for (auto &l : lists)
-    for (auto &w : l) total += w.size();
-  if (total > 0x7fffffffu) return H3D_ELIMIT;
+  int cus = 0;
+  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
+  int n[2][2], blocks[2][2];
+  for (int li = 0; li < 2; ++li) {
+    const int depth = li == 0 ? kb : 1;
+    for (int prec = 0; prec < 2; ++prec) {
+      if (lists[li][prec].size() > 0x3fffffffu) return H3D_ELIMIT;
+      n[li][prec] = (int)lists[li][prec].size();
+    }
+    int rc = occupancy_blocks<0>(cus, depth, n[li][0], &blocks[li][0]);
+    if (rc == H3D_OK) rc = occupancy_blocks<1>(cus, depth, n[li][1], &blocks[li][1]);
+    if (rc != H3D_OK) return rc;
+  }
+  // Each list dealt into the XCD queues in runs of as many consecutive
+  // tiles as an XCD has workgroups, in snake order over the queues (0..7,
+  // 7..0, ...) so that the costlier runs of each sweep spread evenly.
   std::vector<int4> all;
-  all.reserve(total);
-  int off[2][2], n[2][2];
+  std::vector<int> qoff(2 * 2 * 16, 0);
   for (int li = 0; li < 2; ++li)
     for (int prec = 0; prec < 2; ++prec) {
-      off[li][prec] = (int)all.size();
-      n[li][prec] = (int)lists[li][prec].size();
-      all.insert(all.end(), lists[li][prec].begin(), lists[li][prec].end());
+      const std::vector<int4> &w = lists[li][prec];
+      const size_t run = (size_t)std::max(1, (blocks[li][prec] + ENS_QUEUES - 1) / ENS_QUEUES);
+      std::vector<std::vector<int4>> q(ENS_QUEUES);
+      for (size_t r = 0; r * run < w.size(); ++r) {
+        const size_t sweep = r / ENS_QUEUES, pos = r % ENS_QUEUES;
+        const size_t x = sweep % 2 == 0 ? pos : ENS_QUEUES - 1 - pos;
+        const size_t end = std::min(w.size(), (r + 1) * run);
+        q[x].insert(q[x].end(), w.begin() + r * run, w.begin() + end);
+      }
+      int *o = &qoff[(2 * li + prec) * 16];
+      for (int x = 0; x < ENS_QUEUES; ++x) {
+        o[x] = (int)all.size();
+        all.insert(all.end(), q[x].begin(), q[x].end());
+      }
+      o[ENS_QUEUES] = (int)all.size();
     }
-  if (kb == 1)  // the single-step lists are the kb lists themselves
-    for (int prec = 0; prec < 2; ++prec) {
-      off[1][prec] = off[0][prec];
-      n[1][prec] = n[0][prec];
-    }
+  if (all.size() > 0x7fffffffu) return H3D_ELIMIT;
+  const size_t total = all.size();
   if ((int)total > e->work_cap) {
     if (e->d_work) (void)hipFree(e->d_work);
     e->d_work = nullptr;
@@ -241,18 +311,12 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   }
   if (total)
     H3D_TRY(hipMemcpy(e->d_work, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
-  int cus = 0;
-  H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
-  for (int li = 0; li < 2; ++li) {
-    const int depth = li == 0 ? kb : 1;
-    int rc = occupancy_blocks<0>(cus, depth, n[li][0], &e->blocks[li][0]);
-    if (rc == H3D_OK) rc = occupancy_blocks<1>(cus, depth, n[li][1], &e->blocks[li][1]);
-    if (rc != H3D_OK) return rc;
+  H3D_TRY(hipMemcpy(e->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice));
+  for (int li = 0; li < 2; ++li)
     for (int prec = 0; prec < 2; ++prec) {
-      e->off[li][prec] = off[li][prec];
       e->n[li][prec] = n[li][prec];
+      e->blocks[li][prec] = blocks[li][prec];
     }
-  }
   e->zchunk = zchunk;
   e->kb = kb;
   drop_graph(e);
@@ -268,31 +332,39 @@ static int launch_one(h3d_ensemble *e, int li, int parity, hipStream_t s) {
   constexpr size_t lds = ens_lds<KB, PREC>();
   constexpr unsigned threads = ens_waves<PREC>() * 64;
   auto fn = ensemble_step<KB, PREC>;
+  const int q = (2 * li + PREC) * 16;
   hipLaunchKernelGGL(fn, dim3(e->blocks[li][PREC]), dim3(threads), lds, s, e->d_mem,
-                     e->d_work + e->off[li][PREC], n, parity, e->zchunk);
+                     e->d_work, e->d_qoff + q, e->d_qcnt + q, parity, e->zchunk);
   return h3d_launch_status();
 }
 
-template <int KB>
-static int launch_both(h3d_ensemble *e, int li, int parity, hipStream_t s) {
-  const int rc = launch_one<KB, 0>(e, li, parity, s);
-  return rc != H3D_OK ? rc : launch_one<KB, 1>(e, li, parity, s);
-}
-
-// One group: the FP64 and the FP32 kernel over list 0 (every tile, passes
-// of kb steps) or list 1 (every tile, single steps).
-static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
+// The kernel of precision PREC over list 0 (every tile, passes of kb
+// steps) or list 1 (every tile, single steps).
+template <int PREC>
+static int launch_prec(h3d_ensemble *e, int list, int parity, hipStream_t s) {
   const int depth = list == 0 ? e->kb : 1;
-  return depth == 1   ? launch_both<1>(e, list, parity, s)
-         : depth == 2 ? launch_both<2>(e, list, parity, s)
-                      : launch_both<3>(e, list, parity, s);
+  return depth == 1   ? launch_one<1, PREC>(e, list, parity, s)
+         : depth == 2 ? launch_one<2, PREC>(e, list, parity, s)
+                      : launch_one<3, PREC>(e, list, parity, s);
 }
 
-// Passes (launches) in a run of `steps` steps: full groups plus single
-// steps.
+// Passes (launches per precision) in a run of `steps` steps: full groups
+// plus single steps.
 static int ens_passes(const h3d_ensemble *e, int steps) {
   const int full = e->kb > 1 ? steps / e->kb : 0;
   return full + (steps - full * e->kb);
+}
+
+// One group on s: the FP64 kernel, then the FP32 kernel, over list 0
+// (every tile, passes of kb steps) or list 1 (every tile, single steps).
+// The two precisions share no buffers, yet running their chains of
+// launches concurrently on two streams (each kernel filling the other's
+// last round) measured slower: 472 against 487.5 GLUP/s on the 64-member
+// ensemble at z chunks of 64 (MI355X), the interleaved workgroups breaking
+// up the XCD runs that share halo lines in L2.
+static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
+  const int rc = launch_prec<0>(e, list, parity, s);
+  return rc != H3D_OK ? rc : launch_prec<1>(e, list, parity, s);
 }
 
 // Issues the launches of `steps` steps from buffer parity p.
@@ -367,7 +439,12 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
     host[m] = em;
   }
   if (rc != H3D_OK) { ensemble_free(e); return rc; }
-  hipError_t he = hipMalloc(&e->d_mem, sizeof(EMember) * count);
+  // queue offsets and counters; the counters start at zero and every
+  // launch leaves them at zero
+  hipError_t he = hipMalloc(&e->d_qoff, sizeof(int) * 2 * 2 * 16);
+  if (he == hipSuccess) he = hipMalloc(&e->d_qcnt, sizeof(int) * 2 * 2 * 16);
+  if (he == hipSuccess) he = hipMemset(e->d_qcnt, 0, sizeof(int) * 2 * 2 * 16);
+  if (he == hipSuccess) he = hipMalloc(&e->d_mem, sizeof(EMember) * count);
   if (he == hipSuccess)
     he = hipMemcpy(e->d_mem, host.data(), sizeof(EMember) * count, hipMemcpyHostToDevice);
   if (he != hipSuccess) { ensemble_free(e); return (int)he; }
@@ -453,7 +530,7 @@ extern "C" int h3d_ensemble_autotune(h3d_ensemble *e, void *stream, h3d_tune *ou
   float best = 1e30f;
   // every depth is a candidate: FP64-only ensembles gain from depth 2 too
   for (int kb = 1; kb <= ENS_MAXKB && rc == H3D_OK; ++kb)
-    for (int zc : {32, 64, 128, 256}) {
+    for (int zc : {32, 64, 96, 128, 192, 256}) {
       rc = build_work(e, zc, kb);
       if (rc != H3D_OK) break;
       float tg = 0.f;

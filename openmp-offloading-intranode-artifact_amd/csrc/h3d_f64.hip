@@ -361,9 +361,13 @@ extern "C" int h3d_heat_update_boundary_top(double *u_new, const double *u_old,
 // pass of `rem` steps when rem > 0.  A remainder of r steps is one shorter
 // pass, or, without rhs at depths 2 and 3, the last r passes one step
 // deeper when the autotuner has timed both depths on this grid and r
-// deeper passes cost less than a pass: a pass moves the same bytes at any
-// depth and costs about as much, so the deeper passes win when a
-// (K+1)-step pass costs less than (1 + 1/r) K-step passes.
+// deeper passes cost clearly less than a pass: a pass moves the same bytes
+// at any depth and costs about as much, so the deeper passes win when a
+// (K+1)-step pass costs less than (1 + 1/r) K-step passes.  The tuner's
+// figures are the best of back-to-back samples of one shape; inside a run
+// a deeper pass between K-step passes took 5-15% longer (MI355X, 512^3:
+// 1160-1260 us against a tuned figure below 1107 us, beside 740 us for
+// the 3-step pass), so the rule asks for a margin of a quarter of a pass.
 static void run_plan_f64(const h3d_grid *g, bool has_rhs, int steps, int kb, int *full,
                          int *deep, int *rem, h3d_tune *tdeep) {
   *full = steps / kb;
@@ -373,7 +377,7 @@ static void run_plan_f64(const h3d_grid *g, bool has_rhs, int steps, int kb, int
   float t_kb = 0.f, t_deep = 0.f;
   if (h3d_tuned_f64(g, 0, kb, nullptr, &t_kb) == H3D_OK &&
       h3d_tuned_f64(g, 0, kb + 1, tdeep, &t_deep) == H3D_OK &&
-      (float)*rem * (t_deep - t_kb) < t_kb) {
+      (float)*rem * (t_deep - t_kb) < 0.75f * t_kb) {
     *deep = *rem;
     *rem = 0;
   }
