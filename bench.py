@@ -96,25 +96,39 @@ def profile_entry(key, alg_bytes=None):
     copy-kernel calibration, and the kernel trace's average duration.
 
     When that exact shape was not profiled (the autotuner may pick another
-    z chunk from run to run) and alg_bytes is given, the profiled launch of
-    the same kernel instantiation, workgroup size and problem (equal
-    algorithmic bytes) with the nearest grid stands in; its key is returned
-    as "profiled_launch".  None when neither exists."""
+    z chunk or tile from run to run) and alg_bytes is given, a profiled
+    launch of the same problem (equal algorithmic bytes) stands in: first
+    the same kernel instantiation and workgroup size with the nearest grid,
+    else the same kernel family and depth (template arguments but the tile
+    rows and waves) with the nearest grid, marked "same_shape": False.  Its
+    key is returned as "profiled_launch".  None when neither exists."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     try:
         table = json.load(open(path))
     except (OSError, ValueError):
         return None
+    if key in table:
+        return dict(table[key], profiled_launch=key, same_shape=True)
     if alg_bytes is None or "|" not in key:
-        return dict(table[key], profiled_launch=key) if key in table else None
-    kern, grid, wg = key.split("|")
-    near = [(abs(int(k.split("|")[1]) - int(grid)), k) for k, e in table.items()
-            if k.count("|") == 2 and k.split("|")[0] == kern and k.split("|")[2] == wg
-            and e.get("algorithmic_bytes_per_launch") == int(alg_bytes)]
-    if not near:
         return None
-    k = min(near)[1]
-    return dict(table[k], profiled_launch=k)
+    kern, grid, wg = key.split("|")
+
+    def family(kname):
+        # "h3d_pass_f32_kernel<KB, RY, W, MODE, BF16>" -> name, KB and the
+        # arguments after the tile shape; other kernels keep all arguments
+        name, _, args = kname.partition("<")
+        a = [x.strip() for x in args.rstrip(">").split(",")]
+        return (name, a[0], tuple(a[3:])) if "_pass_" in name and len(a) >= 3 else (name, tuple(a))
+
+    same_problem = [(k, e) for k, e in table.items() if k.count("|") == 2
+                    and e.get("algorithmic_bytes_per_launch") == int(alg_bytes)]
+    for exact, match in ((True, lambda k: k.split("|")[0] == kern and k.split("|")[2] == wg),
+                         (False, lambda k: family(k.split("|")[0]) == family(kern))):
+        near = [(abs(int(k.split("|")[1]) - int(grid)), k) for k, _ in same_problem if match(k)]
+        if near:
+            k = min(near)[1]
+            return dict(table[k], profiled_launch=k, same_shape=exact)
+    return None
 
 
 def traffic_ratio(key, alg_bytes):
@@ -123,7 +137,8 @@ def traffic_ratio(key, alg_bytes):
         return None
     return {"bytes_per_launch": e["bytes_per_launch"],
             "ratio_to_algorithmic": round(e["bytes_per_launch"] / alg_bytes, 4),
-            "profiled_launch": e["profiled_launch"], "profile": e.get("source")}
+            "profiled_launch": e["profiled_launch"], "same_shape": e["same_shape"],
+            "profile": e.get("source")}
 
 
 def pass_key(kernel, kb, rows, waves, zc, m, extra=""):
@@ -792,7 +807,7 @@ def main():
         pa = alg_bytes / (prof["trace_avg_us"] * 1e-6) / 1e9
         profiled = {"kernel_us_per_launch": prof["trace_avg_us"], "achieved": round(pa, 1),
                     "frac": round(pa / HBM_PEAK_GBS, 4), "launch": prof["profiled_launch"],
-                    "profile": prof.get("source")}
+                    "same_shape": prof["same_shape"], "profile": prof.get("source")}
 
     result = {
         "metric": METRIC, "value": round(glups, 3), "unit": "GLUP/s", "n_gpus": 1,
