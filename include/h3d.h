@@ -120,7 +120,7 @@ int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
  * has at least r passes and h3d_autotune has timed depths K and K+1 on
  * this grid (it does so when it picks K), the last r passes are K+1 steps
  * deep instead (the tuner's K+1 shape) if by those times that saves more
- * than a quarter of a K-step pass, one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
+ * than half a K-step pass, one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
  * passes (up to 1024 of them; longer runs replay a graph of 1024 K-step
  * passes) in a hipGraph and launches it.  *ms receives
  * the device time of the loop measured with HIP events on `stream`;
@@ -290,7 +290,7 @@ int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,
  * is not used); results are cached per
  * (kernel, grid, mode, kappa format, fixed_kblock).  kind: 0 = FP64 step
  * or pass (mode 0: no rhs, mode 1: with rhs, which the shapes' LDS must
- * hold; kblock chosen too, 1..3, unless fixed_kblock 1..4 is given), 1 =
+ * hold; kblock chosen too, 1..4, unless fixed_kblock 1..4 is given), 1 =
  * FP32 pass (kblock chosen too, 1..4, unless fixed_kblock 1..8 is given).
  * The figure of merit is time per step.  The caller's buffers are not
  * modified. */

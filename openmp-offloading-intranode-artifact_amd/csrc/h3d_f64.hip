@@ -157,7 +157,10 @@ int launch_pass_kb(int ry, int waves, double *uo, const double *ui, const double
                                                    zchunk, s, occ, tiles);
   H3D_P(2, 4) H3D_P(2, 8) H3D_P(4, 4) H3D_P(4, 8)
   // 32 rows as 16 waves of 2: four waves per SIMD where rows 4 x 8 waves
-  // run two (160 KB of LDS at 3 steps; 208 KB at 4 would not fit)
+  // run two (128 KB of LDS at 3 steps).  At 4 steps the tile takes the
+  // whole 160 KB and spills past the 128 VGPRs of a 16-wave workgroup; it
+  // measured 1430-1600 us per pass at 512^3 against 1060 us for rows 4 x
+  // 8 waves, so it is not offered.
   if constexpr (KB <= 3) {
     H3D_P(2, 16)
   }
@@ -364,10 +367,13 @@ extern "C" int h3d_heat_update_boundary_top(double *u_new, const double *u_old,
 // deeper passes cost clearly less than a pass: a pass moves the same bytes
 // at any depth and costs about as much, so the deeper passes win when a
 // (K+1)-step pass costs less than (1 + 1/r) K-step passes.  The tuner's
-// figures are the best of back-to-back samples of one shape; inside a run
-// a deeper pass between K-step passes took 5-15% longer (MI355X, 512^3:
-// 1160-1260 us against a tuned figure below 1107 us, beside 740 us for
-// the 3-step pass), so the rule asks for a margin of a quarter of a pass.
+// figures are the best of back-to-back samples of one shape, and a deeper
+// pass runs slower than that between K-step passes: on MI355X at 512^3
+// the 4-step pass on 8 waves x 4 rows took 1058-1172 us interleaved with
+// 3-step passes (720-745 us), and back to back it drifted from 1080 up to
+// 1300-1420 us and down to 1020-1060 us over twenty launches.  So the rule
+// asks for a saving of half a pass: a 1-step remainder rides on one deeper
+// pass, a 2-step remainder at depth 3 takes the short pass.
 static void run_plan_f64(const h3d_grid *g, bool has_rhs, int steps, int kb, int *full,
                          int *deep, int *rem, h3d_tune *tdeep) {
   *full = steps / kb;
@@ -377,7 +383,7 @@ static void run_plan_f64(const h3d_grid *g, bool has_rhs, int steps, int kb, int
   float t_kb = 0.f, t_deep = 0.f;
   if (h3d_tuned_f64(g, 0, kb, nullptr, &t_kb) == H3D_OK &&
       h3d_tuned_f64(g, 0, kb + 1, tdeep, &t_deep) == H3D_OK &&
-      (float)*rem * (t_deep - t_kb) < 0.75f * t_kb) {
+      (float)*rem * (t_deep - t_kb) < 0.5f * t_kb) {
     *deep = *rem;
     *rem = 0;
   }

@@ -80,9 +80,13 @@ int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, in
   // (they share the bandwidth), yet not in proportion.  Measured on MI355X
   // at 512^3 (3-step FP64 pass, 180 tiles): z chunks of 128, 103, 86 and
   // 73 planes run in the ratios 1 : 1.02 : 1.06 : 1.07 this predicts, and
-  // 2 chunks (1.4 rounds) far slower.  Chunks are at most 256 planes: a
-  // whole 510-plane march per workgroup ran at half the rate of 128-plane
-  // chunks there, beyond what the model accounts for.
+  // 2 chunks (1.4 rounds) far slower.  Chunks are at most 256 planes,
+  // except one chunk of the whole z range when every tile then has a
+  // resident slot (one round, no warm-up planes re-read): at 1024^3 marches
+  // of 512 and 1024 planes over several rounds ran 8-22% slower than the
+  // model predicts, while at 512^3 the 4-step pass on 8 waves x 4 rows
+  // (220 tiles) ran 985 us per pass as one whole-z round against 1009 us in
+  // two chunks of 256 and 1058 us in eight of 64, as predicted.
   struct C {
     double cost;
     long long rounds;
@@ -95,7 +99,7 @@ int h3d_round_zchunks(int planes, int kblock, long long tiles, int wg_per_cu, in
     if (z == last) continue;
     last = z;
     if (c > 1 && z < 4 * kblock) break;  // warm-up would exceed half the work
-    if (z > 256) continue;
+    if (z > 256 && !(c == 1 && tiles <= slots)) continue;
     const long long wgs = tiles * ((planes + z - 1) / z);
     const long long rounds = (wgs + slots - 1) / slots;
     const double march = (double)(z + 2 * kblock + 8);
@@ -173,8 +177,8 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
   std::vector<std::pair<float, h3d_tune>> cand;  // (ms per step, shape)
   if (rc == H3D_OK && kind == 0) {
     const int planes = g->nz - 2;
-    // depths 1..3 when the tuner chooses; a fixed depth (up to 4) alone
-    for (int kbk = 1; kbk <= (fixed_kblock ? fixed_kblock : 3) && rc == H3D_OK; ++kbk) {
+    // depths 1..4 when the tuner chooses; a fixed depth alone
+    for (int kbk = 1; kbk <= (fixed_kblock ? fixed_kblock : 4) && rc == H3D_OK; ++kbk) {
       if (fixed_kblock && kbk != fixed_kblock) continue;
       // single steps: whole z range first (on large grids the chunk-edge
       // re-reads cost more than the extra parallelism of short chunks
