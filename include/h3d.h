@@ -318,9 +318,9 @@ typedef struct h3d_member {
 typedef struct h3d_ensemble h3d_ensemble;
 
 /* Builds the tile work lists on `device_id` and allocates the FP32
- * members' residual fields (zeroed).  Tiles are fixed in x and y (32 rows;
- * FP64 members in 1024-thread and FP32 members in 512-thread workgroups,
- * each precision its own persistent kernel).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
+ * members' residual fields (zeroed).  Tiles are fixed in x and y (1024-
+ * thread workgroups of 32 rows for FP64 members and 48 rows for FP32
+ * members, each precision its own persistent kernel).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
  * depth (planes per work item, default 64) and tune->kblock the
  * temporal-blocking depth (steps fused per pass of every member, 1..3,
  * default 1; H3D_ELIMIT above). */
