@@ -654,20 +654,19 @@ def leg_cfg5(bw, stream):
     b64 = sum(24.0 / k64 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
     b32 = sum(16.0 / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
     gbs = (b64 + b32) * steps / sec / 1e9
-    # one launch per precision and group of kblock steps: the pass over all
-    # members of that precision (profile keys by kernel name: the grid is
-    # the device's resident workgroups)
+    # one launch per group of kblock steps: the pass over every member
+    # (profile key by kernel name: the grid is the device's resident
+    # workgroups)
     c64 = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
     c32 = cells - c64
-    traffic = {}
-    for prec, alg in ((0, 24.0 * c64), (1, 16.0 * c32)):
-        key = f"ensemble_step<{et.kblock}, {prec}>"
-        note_launch(f"ensemble_prec{prec}", key, alg)
-        traffic["fp64_members" if prec == 0 else "fp32_members"] = {
-            "launch": key, "traffic": traffic_ratio(key, alg)}
+    alg = 24.0 * c64 + 16.0 * c32
+    key = f"ensemble_step<{et.kblock}>"
+    note_launch("ensemble", key, alg)
+    traffic = {"launch": key, "algorithmic_bytes_per_launch": alg,
+               "traffic": traffic_ratio(key, alg)}
     return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
             "cells": int(cells), "steps_timed": steps, "graph": True, "zchunk": et.zchunk,
-            "kblock": et.kblock, "launches": traffic,
+            "kblock": et.kblock, "launch": traffic,
             "ms_per_step": round(1e3 * sec / steps, 5),
             "work_items": nwork.value, "workgroups": nblk.value,
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),

@@ -320,14 +320,15 @@ typedef struct h3d_ensemble h3d_ensemble;
 /* Builds the tile work lists on `device_id` and allocates the FP32
  * members' residual fields (zeroed).  Tiles are fixed in x and y (1024-
  * thread workgroups of 32 rows for FP64 members and 48 rows for FP32
- * members, each precision its own persistent kernel).  tune may be NULL; tune->zchunk, if non-zero, is the z-march
+ * members; one persistent kernel serves both).  tune may be NULL;
+ * tune->zchunk, if non-zero, is the z-march
  * depth (planes per work item, default 64) and tune->kblock the
  * temporal-blocking depth (steps fused per pass of every member, 1..3,
  * default 1; H3D_ELIMIT above). */
 int h3d_ensemble_create(h3d_ensemble **out, const h3d_member *members,
                         int count, const h3d_tune *tune, int device_id);
 /* `steps` time steps of every member.  With kblock K > 1, each group of K
- * steps is one launch per precision of K-step passes for every member
+ * steps is one persistent launch of K-step passes for every member
  * (FP64 members bit-identical to h3d_pass_f64); steps past the last full
  * group are single-step launches.  graph != 0 captures the run's launches into one
  * hipGraph (kept and reused while the step count and starting parity

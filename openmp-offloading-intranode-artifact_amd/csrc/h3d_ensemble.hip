@@ -7,9 +7,8 @@
 // int4 {member, tx, ty, tz} per tile, longest chunks first, split into
 // one queue per XCD, and a fixed grid of workgroups -- as many as are
 // resident on the device at once -- takes tiles from the queues until all
-// are done (ensemble_step).  FP64 and FP32 members have a list and a
-// kernel each, launched one after the other, so each kernel runs at the
-// occupancy of its own tile:
+// are done (ensemble_step).  One persistent launch per pass serves FP64
+// and FP32 members alike, each tile with its precision's code and shape:
 //  - FP64 members: 1024-thread workgroups of 32 rows (16 waves x 2 rows),
 //    the FP64 step tile for single steps (bit-identical to h3d_step_f64)
 //    and the FP64 pass tile for passes (bit-identical to h3d_pass_f64);
@@ -21,12 +20,17 @@
 //    arithmetic).  Against 512-thread workgroups of 32 rows (8 waves x 4
 //    rows) the ensemble ran 512.5 vs 491.6 GLUP/s (16 x 2: 505.5; 16 x 4
 //    spills 19 VGPRs with the residual field).  Inside a persistent kernel
-//    the LDS tile
-//    measured 8% faster than the wave-independent single-step tile, so it
-//    is used for kblock = 1 too.
+//    the LDS tile measured 8% faster than the wave-independent single-step
+//    tile, so it is used for kblock = 1 too.
+// Both tiles are 1024-thread workgroups that fill a CU alone (the FP64
+// pass tile's 128 KB of LDS), so one kernel holds either at the occupancy
+// it would have alone, and the FP64 and FP32 tiles share the queues: one
+// launch, one tail per pass.  As two kernels, one per precision, one after
+// the other, the 64-member ensemble ran 509 GLUP/s against 518 (and 500
+// against 514 on another box, with the tiles ordered by weighted cost).
 //
-// With kblock = K > 1 a group of K steps is one launch per precision in
-// which every tile takes a K-step pass.  Steps beyond the last full group
+// With kblock = K > 1 a group of K steps is one launch in which every tile
+// takes a K-step pass.  Steps beyond the last full group
 // are single-step launches over every tile.  Every member flips its
 // ping-pong buffers once per group (h3d_ensemble_where).  The launches of
 // a run are captured into one hipGraph on request.
@@ -37,19 +41,14 @@
 
 namespace {
 
-// FP64 and FP32 members run in separate persistent kernels, each with the
-// workgroup shape and LDS of its own tile, so neither is held to the
-// other's occupancy.
-constexpr int ENS64_WAVES = 16; // FP64 tiles: 16 waves x 2 rows = 32 rows
+// Every tile is a workgroup of ENS_WAVES waves: FP64 members 2 rows per
+// lane (32 rows), FP32 members 3 (48 rows).
+constexpr int ENS_WAVES = 16;
 constexpr int ENS64_ROWS = 2;   // rows per lane of the FP64 step tile
 constexpr int ENS64_PRY = 2;    // rows per lane of the FP64 pass tile
-constexpr int ENS32_WAVES = 16; // FP32 tiles: 16 waves x 3 rows = 48 rows
-constexpr int ENS32_RY = 3;
+constexpr int ENS32_RY = 3;     // rows per lane of the FP32 pass tile
 constexpr int ENS_ZCHUNK = 64;  // default z-march depth
 constexpr int ENS_MAXKB = 3;    // deepest pass fused inside the ensemble
-
-template <int PREC>
-constexpr int ens_waves() { return PREC ? ENS32_WAVES : ENS64_WAVES; }
 
 struct EMember {
   void *ua, *ub;
@@ -62,26 +61,25 @@ struct EMember {
   int precision;
 };
 
-// LDS of the (depth, precision) kernel in bytes: the tile's, then one
-// int for the tile index the workgroup has taken from its queue (kept at
-// the end, so the tile's doubles stay 8-byte aligned).
-template <int KB, int PREC>
+// LDS of the depth-KB kernel in bytes: the larger of the two tiles',
+// then one int for the tile index the workgroup has taken from its queue
+// (kept at the end, so the tile's doubles stay 8-byte aligned).
+template <int KB>
 constexpr size_t ens_tile_lds() {
-  if constexpr (PREC == 1)
-    return sizeof(float) * (size_t)pass_lds_floats(KB, ENS32_RY, ENS32_WAVES);
-  else
-    return KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS64_PRY, ENS64_WAVES)
-                  : 0;
+  const size_t f32 = sizeof(float) * (size_t)pass_lds_floats(KB, ENS32_RY, ENS_WAVES);
+  const size_t f64 =
+      KB > 1 ? sizeof(double) * (size_t)pass64_lds_doubles(KB, ENS64_PRY, ENS_WAVES) : 0;
+  return f32 > f64 ? f32 : f64;
 }
-template <int KB, int PREC>
-constexpr size_t ens_lds() { return ens_tile_lds<KB, PREC>() + 16; }
+template <int KB>
+constexpr size_t ens_lds() { return ens_tile_lds<KB>() + 16; }
 
 // XCD queues of a work list: workgroup b runs on XCD b mod 8 and takes
 // its tiles from that XCD's queue first.
 constexpr int ENS_QUEUES = 8;
 
-// Tiles of members of precision PREC (0 FP64, 1 FP32) at pass depth KB;
-// parity: which buffer every member reads this launch.
+// Tiles of every member at pass depth KB; parity: which buffer every
+// member reads this launch.
 //
 // The list is split into ENS_QUEUES queues, work[qoff[x] .. qoff[x+1]),
 // one per XCD, each costliest tile first.  A workgroup takes the next tile
@@ -94,12 +92,12 @@ constexpr int ENS_QUEUES = 8;
 // chunk march together under one L2 and share their halo lines there.
 // The last workgroup to finish zeroes the counters for the next launch,
 // so a launch needs no reset of its own (none inside a captured graph).
-template <int KB, int PREC>
-__global__ void __launch_bounds__(ens_waves<PREC>() * 64)
+template <int KB>
+__global__ void __launch_bounds__(ENS_WAVES * 64)
 ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
               const int *__restrict__ qoff, int *qcnt, int parity, int zchunk) {
   extern __shared__ float lds[];
-  int *grabbed = reinterpret_cast<int *>(reinterpret_cast<char *>(lds) + ens_tile_lds<KB, PREC>());
+  int *grabbed = reinterpret_cast<int *>(reinterpret_cast<char *>(lds) + ens_tile_lds<KB>());
   const int home = (int)(blockIdx.x % ENS_QUEUES);
   for (int q = 0; q < ENS_QUEUES; ++q) {
     const int x = (home + q) % ENS_QUEUES;
@@ -107,17 +105,19 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
     for (;;) {
       if (threadIdx.x == 0) *grabbed = atomicAdd(&qcnt[x], 1);
       __syncthreads();
-      const int t = *grabbed;
+      // the same value in every lane: read as a scalar, so that the tile
+      // descriptor and the member record load into scalar registers
+      const int t = __builtin_amdgcn_readfirstlane(*grabbed);
       __syncthreads();  // every thread has read it before it is rewritten
       if (t >= n) break;  // workgroup-uniform
       const int4 d = work[lo + t];
       const EMember m = mem[d.x];
-      if constexpr (PREC == 0) {
+      if (m.precision == 0) {  // workgroup-uniform
         const double *in = static_cast<const double *>(parity ? m.ub : m.ua);
         double *out = static_cast<double *>(parity ? m.ua : m.ub);
         if constexpr (KB > 1) {
           const int zb = 1 + d.w * zchunk;
-          pass_f64_tile<KB, ENS64_PRY, ENS64_WAVES, false>(
+          pass_f64_tile<KB, ENS64_PRY, ENS_WAVES, false>(
               reinterpret_cast<double *>(lds), out, in, nullptr,
               static_cast<const double *>(m.kap), m.nx, m.ny, m.nz, m.sy, m.sz, m.factor, zb,
               min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
@@ -133,7 +133,7 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
         const unsigned short *cin = parity ? m.cb : m.ca;
         unsigned short *cout = parity ? m.ca : m.cb;
         const int zb = 1 + d.w * zchunk;
-        pass_f32_tile<KB, ENS32_RY, ENS32_WAVES, 2, false>(
+        pass_f32_tile<KB, ENS32_RY, ENS_WAVES, 2, false>(
             lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
             m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
             min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
@@ -169,15 +169,15 @@ struct h3d_ensemble {
     int tiles_x[ENS_MAXKB + 1], tiles_y[ENS_MAXKB + 1], planes, precision;
   };
   std::vector<Tiles> tiles;  // per member
-  // Work lists in one device buffer, per list (0: every tile at depth kb,
-  // 1: every tile at depth 1, for the steps past the last full group) and
-  // precision, with the persistent grid of each kernel.
+  // Work lists in one device buffer (0: every tile at depth kb, 1: every
+  // tile at depth 1, for the steps past the last full group), with the
+  // persistent grid of each list's kernel.
   int4 *d_work;
   int work_cap;
-  int n[2][2], blocks[2][2];
-  // per (list, precision): the ENS_QUEUES + 1 queue offsets into d_work
-  // (d_qoff + (2 * list + precision) * 16) and the queue counters plus the
-  // finished-workgroup count (d_qcnt, same layout; zero between launches)
+  int n[2], blocks[2];
+  // per list: the ENS_QUEUES + 1 queue offsets into d_work (d_qoff +
+  // list * 16) and the queue counters plus the finished-workgroup count
+  // (d_qcnt, same layout; zero between launches)
   int *d_qoff;
   int *d_qcnt;
   std::vector<unsigned short *> comp;  // owned FP16 residual fields
@@ -209,32 +209,30 @@ static void ensemble_free(h3d_ensemble *e) {
 
 // Kernels whose LDS exceeds the 64 KB default need the attribute raised on
 // each device before they are sized or launched.
-template <int KB, int PREC>
+template <int KB>
 static int allow_lds() {
-  if (ens_lds<KB, PREC>() <= 64 * 1024) return H3D_OK;
+  if (ens_lds<KB>() <= 64 * 1024) return H3D_OK;
   static std::atomic<unsigned long long> done{0};
-  return h3d_allow_big_lds((const void *)ensemble_step<KB, PREC>, &done);
+  return h3d_allow_big_lds((const void *)ensemble_step<KB>, &done);
 }
 
-// Persistent grid of the (depth, precision) kernel for nwork tiles: as
-// many workgroups as are resident on the device at once.
-template <int KB, int PREC>
+// Persistent grid of the depth-kb kernel for nwork tiles: as many
+// workgroups as are resident on the device at once.
+template <int KB>
 static int occupancy_one(int cus, int nwork, int *blocks) {
-  const int lrc = allow_lds<KB, PREC>();
+  const int lrc = allow_lds<KB>();
   if (lrc != H3D_OK) return lrc;
   int per_cu = 0;
   H3D_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-      &per_cu, (const void *)ensemble_step<KB, PREC>, ens_waves<PREC>() * 64,
-      ens_lds<KB, PREC>()));
+      &per_cu, (const void *)ensemble_step<KB>, ENS_WAVES * 64, ens_lds<KB>()));
   *blocks = std::max(1, std::min(std::max(nwork, 1), cus * std::max(per_cu, 1)));
   return H3D_OK;
 }
 
-template <int PREC>
 static int occupancy_blocks(int cus, int kb, int nwork, int *blocks) {
-  return kb == 1   ? occupancy_one<1, PREC>(cus, nwork, blocks)
-         : kb == 2 ? occupancy_one<2, PREC>(cus, nwork, blocks)
-                   : occupancy_one<3, PREC>(cus, nwork, blocks);
+  return kb == 1   ? occupancy_one<1>(cus, nwork, blocks)
+         : kb == 2 ? occupancy_one<2>(cus, nwork, blocks)
+                   : occupancy_one<3>(cus, nwork, blocks);
 }
 
 // (Re)builds the work lists for z chunks of `zchunk` planes and passes of
@@ -243,16 +241,18 @@ static int occupancy_blocks(int cus, int kb, int nwork, int *blocks) {
 static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   if (zchunk < 1 || kb < 1 || kb > ENS_MAXKB) return H3D_EARG;
   struct Job { long long cost; int4 d; };
-  auto list = [&](int depth, int prec) {
+  auto list = [&](int depth) {
     std::vector<Job> jobs;
     for (int m = 0; m < e->count; ++m) {
       const h3d_ensemble::Tiles &t = e->tiles[m];
-      if (t.precision != prec) continue;
       const int tx_n = t.tiles_x[depth], ty_n = t.tiles_y[depth];
       const int chunks = (t.planes + zchunk - 1) / zchunk;
       for (int tz = 0; tz < chunks; ++tz) {
         const int planes = std::min(zchunk, t.planes - tz * zchunk);
-        const long long cost = planes;  // one tile's planes
+        // a tile's planes, weighted by the measured time of one plane
+        // iteration: 2.4 us for the 48-row FP32 pass tile against 1.7 us
+        // for the 32-row FP64 tile (depth 3, 64-member ensemble, MI355X)
+        const long long cost = (long long)planes * (t.precision ? 7 : 5);
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
       }
@@ -263,47 +263,40 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
     for (size_t j = 0; j < jobs.size(); ++j) w[j] = jobs[j].d;
     return w;
   };
-  std::vector<int4> lists[2][2];
-  for (int prec = 0; prec < 2; ++prec) {
-    lists[0][prec] = list(kb, prec);
-    lists[1][prec] = kb > 1 ? list(1, prec) : lists[0][prec];
-  }
+  std::vector<int4> lists[2];
+  lists[0] = list(kb);
+  lists[1] = kb > 1 ? list(1) : lists[0];
   int cus = 0;
   H3D_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, e->device));
-  int n[2][2], blocks[2][2];
+  int n[2], blocks[2];
   for (int li = 0; li < 2; ++li) {
-    const int depth = li == 0 ? kb : 1;
-    for (int prec = 0; prec < 2; ++prec) {
-      if (lists[li][prec].size() > 0x3fffffffu) return H3D_ELIMIT;
-      n[li][prec] = (int)lists[li][prec].size();
-    }
-    int rc = occupancy_blocks<0>(cus, depth, n[li][0], &blocks[li][0]);
-    if (rc == H3D_OK) rc = occupancy_blocks<1>(cus, depth, n[li][1], &blocks[li][1]);
+    if (lists[li].size() > 0x3fffffffu) return H3D_ELIMIT;
+    n[li] = (int)lists[li].size();
+    const int rc = occupancy_blocks(cus, li == 0 ? kb : 1, n[li], &blocks[li]);
     if (rc != H3D_OK) return rc;
   }
   // Each list dealt into the XCD queues in runs of as many consecutive
   // tiles as an XCD has workgroups, in snake order over the queues (0..7,
   // 7..0, ...) so that the costlier runs of each sweep spread evenly.
   std::vector<int4> all;
-  std::vector<int> qoff(2 * 2 * 16, 0);
-  for (int li = 0; li < 2; ++li)
-    for (int prec = 0; prec < 2; ++prec) {
-      const std::vector<int4> &w = lists[li][prec];
-      const size_t run = (size_t)std::max(1, (blocks[li][prec] + ENS_QUEUES - 1) / ENS_QUEUES);
-      std::vector<std::vector<int4>> q(ENS_QUEUES);
-      for (size_t r = 0; r * run < w.size(); ++r) {
-        const size_t sweep = r / ENS_QUEUES, pos = r % ENS_QUEUES;
-        const size_t x = sweep % 2 == 0 ? pos : ENS_QUEUES - 1 - pos;
-        const size_t end = std::min(w.size(), (r + 1) * run);
-        q[x].insert(q[x].end(), w.begin() + r * run, w.begin() + end);
-      }
-      int *o = &qoff[(2 * li + prec) * 16];
-      for (int x = 0; x < ENS_QUEUES; ++x) {
-        o[x] = (int)all.size();
-        all.insert(all.end(), q[x].begin(), q[x].end());
-      }
-      o[ENS_QUEUES] = (int)all.size();
+  std::vector<int> qoff(2 * 16, 0);
+  for (int li = 0; li < 2; ++li) {
+    const std::vector<int4> &w = lists[li];
+    const size_t run = (size_t)std::max(1, (blocks[li] + ENS_QUEUES - 1) / ENS_QUEUES);
+    std::vector<std::vector<int4>> q(ENS_QUEUES);
+    for (size_t r = 0; r * run < w.size(); ++r) {
+      const size_t sweep = r / ENS_QUEUES, pos = r % ENS_QUEUES;
+      const size_t x = sweep % 2 == 0 ? pos : ENS_QUEUES - 1 - pos;
+      const size_t end = std::min(w.size(), (r + 1) * run);
+      q[x].insert(q[x].end(), w.begin() + r * run, w.begin() + end);
     }
+    int *o = &qoff[li * 16];
+    for (int x = 0; x < ENS_QUEUES; ++x) {
+      o[x] = (int)all.size();
+      all.insert(all.end(), q[x].begin(), q[x].end());
+    }
+    o[ENS_QUEUES] = (int)all.size();
+  }
   if (all.size() > 0x7fffffffu) return H3D_ELIMIT;
   const size_t total = all.size();
   if ((int)total > e->work_cap) {
@@ -316,59 +309,41 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
   if (total)
     H3D_TRY(hipMemcpy(e->d_work, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
   H3D_TRY(hipMemcpy(e->d_qoff, qoff.data(), sizeof(int) * qoff.size(), hipMemcpyHostToDevice));
-  for (int li = 0; li < 2; ++li)
-    for (int prec = 0; prec < 2; ++prec) {
-      e->n[li][prec] = n[li][prec];
-      e->blocks[li][prec] = blocks[li][prec];
-    }
+  for (int li = 0; li < 2; ++li) {
+    e->n[li] = n[li];
+    e->blocks[li] = blocks[li];
+  }
   e->zchunk = zchunk;
   e->kb = kb;
   drop_graph(e);
   return H3D_OK;
 }
 
-template <int KB, int PREC>
+template <int KB>
 static int launch_one(h3d_ensemble *e, int li, int parity, hipStream_t s) {
-  const int n = e->n[li][PREC];
-  if (n == 0) return H3D_OK;
-  const int lrc = allow_lds<KB, PREC>();
+  if (e->n[li] == 0) return H3D_OK;
+  const int lrc = allow_lds<KB>();
   if (lrc != H3D_OK) return lrc;
-  constexpr size_t lds = ens_lds<KB, PREC>();
-  constexpr unsigned threads = ens_waves<PREC>() * 64;
-  auto fn = ensemble_step<KB, PREC>;
-  const int q = (2 * li + PREC) * 16;
-  hipLaunchKernelGGL(fn, dim3(e->blocks[li][PREC]), dim3(threads), lds, s, e->d_mem,
-                     e->d_work, e->d_qoff + q, e->d_qcnt + q, parity, e->zchunk);
+  hipLaunchKernelGGL(ensemble_step<KB>, dim3(e->blocks[li]), dim3(ENS_WAVES * 64),
+                     ens_lds<KB>(), s, e->d_mem, e->d_work, e->d_qoff + li * 16,
+                     e->d_qcnt + li * 16, parity, e->zchunk);
   return h3d_launch_status();
 }
 
-// The kernel of precision PREC over list 0 (every tile, passes of kb
-// steps) or list 1 (every tile, single steps).
-template <int PREC>
-static int launch_prec(h3d_ensemble *e, int list, int parity, hipStream_t s) {
-  const int depth = list == 0 ? e->kb : 1;
-  return depth == 1   ? launch_one<1, PREC>(e, list, parity, s)
-         : depth == 2 ? launch_one<2, PREC>(e, list, parity, s)
-                      : launch_one<3, PREC>(e, list, parity, s);
-}
-
-// Passes (launches per precision) in a run of `steps` steps: full groups
-// plus single steps.
+// Passes (launches) in a run of `steps` steps: full groups plus single
+// steps.
 static int ens_passes(const h3d_ensemble *e, int steps) {
   const int full = e->kb > 1 ? steps / e->kb : 0;
   return full + (steps - full * e->kb);
 }
 
-// One group on s: the FP64 kernel, then the FP32 kernel, over list 0
-// (every tile, passes of kb steps) or list 1 (every tile, single steps).
-// The two precisions share no buffers, yet running their chains of
-// launches concurrently on two streams (each kernel filling the other's
-// last round) measured slower: 472 against 487.5 GLUP/s on the 64-member
-// ensemble at z chunks of 64 (MI355X), the interleaved workgroups breaking
-// up the XCD runs that share halo lines in L2.
+// One group on s: the kernel over list 0 (every tile, passes of kb steps)
+// or list 1 (every tile, single steps).
 static int ens_launch(h3d_ensemble *e, int list, int parity, hipStream_t s) {
-  const int rc = launch_prec<0>(e, list, parity, s);
-  return rc != H3D_OK ? rc : launch_prec<1>(e, list, parity, s);
+  const int depth = list == 0 ? e->kb : 1;
+  return depth == 1   ? launch_one<1>(e, list, parity, s)
+         : depth == 2 ? launch_one<2>(e, list, parity, s)
+                      : launch_one<3>(e, list, parity, s);
 }
 
 // Issues the launches of `steps` steps from buffer parity p.
@@ -421,9 +396,9 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
       // pass tile (FP32 at any depth, FP64 from depth 2) keeps
       // (64 - 2d) x (RY x waves - 2d) of its cells
       const bool step64 = x.precision == 0 && d == 1;
-      const int rows = x.precision ? ENS32_RY * ENS32_WAVES : ENS64_PRY * ENS64_WAVES;
+      const int rows = (x.precision ? ENS32_RY : ENS64_PRY) * ENS_WAVES;
       const int OX = step64 ? 64 : 64 - 2 * d;
-      const int OY = step64 ? ENS64_ROWS * ENS64_WAVES : rows - 2 * d;
+      const int OY = step64 ? ENS64_ROWS * ENS_WAVES : rows - 2 * d;
       t.tiles_x[d] = (x.g.nx - 2 + OX - 1) / OX;
       t.tiles_y[d] = (x.g.ny - 2 + OY - 1) / OY;
     }
@@ -445,9 +420,9 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
   if (rc != H3D_OK) { ensemble_free(e); return rc; }
   // queue offsets and counters; the counters start at zero and every
   // launch leaves them at zero
-  hipError_t he = hipMalloc(&e->d_qoff, sizeof(int) * 2 * 2 * 16);
-  if (he == hipSuccess) he = hipMalloc(&e->d_qcnt, sizeof(int) * 2 * 2 * 16);
-  if (he == hipSuccess) he = hipMemset(e->d_qcnt, 0, sizeof(int) * 2 * 2 * 16);
+  hipError_t he = hipMalloc(&e->d_qoff, sizeof(int) * 2 * 16);
+  if (he == hipSuccess) he = hipMalloc(&e->d_qcnt, sizeof(int) * 2 * 16);
+  if (he == hipSuccess) he = hipMemset(e->d_qcnt, 0, sizeof(int) * 2 * 16);
   if (he == hipSuccess) he = hipMalloc(&e->d_mem, sizeof(EMember) * count);
   if (he == hipSuccess)
     he = hipMemcpy(e->d_mem, host.data(), sizeof(EMember) * count, hipMemcpyHostToDevice);
@@ -564,8 +539,8 @@ extern "C" int h3d_ensemble_where(const h3d_ensemble *e, int member, int *in_b) 
 extern "C" int h3d_ensemble_info(const h3d_ensemble *e, int *nwork,
                                  int *nblocks, long long *steps_done) {
   if (!e) return H3D_EARG;
-  if (nwork) *nwork = e->n[0][0] + e->n[0][1];
-  if (nblocks) *nblocks = e->blocks[0][0] + e->blocks[0][1];
+  if (nwork) *nwork = e->n[0];
+  if (nblocks) *nblocks = e->blocks[0];
   if (steps_done) *steps_done = e->done;
   return H3D_OK;
 }
