@@ -136,6 +136,22 @@ def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
     them, and each FP64 member its plain run, over runs whose lengths are
     not multiples of kb."""
     specs = [make_member(gpu, d, m, 13, p) for m, (d, p) in enumerate(SPECS)]
+    check_members_over_runs(gpu, oracle, specs, kb, graph)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_single_precision_ensembles_equal_standalone(gpu, oracle, prec):
+    """One persistent kernel serves FP64 and FP32 tiles; an ensemble whose
+    members all have one precision takes only that path, and each member
+    still equals its standalone run (3-step passes and single steps, from
+    a graph)."""
+    specs = [make_member(gpu, d, m, 29, prec) for m, (d, _) in enumerate(SPECS)]
+    check_members_over_runs(gpu, oracle, specs, 3, 1)
+
+
+def check_members_over_runs(gpu, oracle, specs, kb, graph):
+    """Runs of 3, 4 and kb steps at pass depth kb, then every member
+    against the oracle."""
     mems, keep = [], []
     for sp in specs:
         da, db, dk = (gpu.DeviceArray.from_host(x) for x in (sp["u"], sp["u"], sp["k"]))
