@@ -395,6 +395,7 @@ def leg_cfg3(bw, stream):
         f.close()
     deep = {f"ghost{K}": slab_deep(n, planes - 2, gks[1], K, factor, steps, bw, stream)
             for K in (2, 3)}
+    deep["fp32c_ghost3"] = slab_deep_f32(n, planes - 2, gks[1], 3, factor, steps, bw, stream)
     # the interior launch (planes 2..nz-3) dominates a step
     key = march_key("h3d_march_f64", t.rows, t.waves, t.zchunk, n, n, planes - 4, ", false")
     alg = 24.0 * (n - 2) ** 2 * (planes - 4)
@@ -453,6 +454,49 @@ def slab_deep(n, owned, gk_owned, K, factor, steps, bw, stream):
             "frac_of_stream": round(gbs / bw, 4), "slab": [n, n, planes],
             "band_bytes": K * n * n * 8, "tune": [t.rows, t.waves, t.zchunk, t.kblock],
             "nonfinite": float(st[3]), "launch": key, "traffic": traffic_ratio(key, alg)}
+
+
+def slab_deep_f32(n, owned, gk_owned, K, factor, steps, bw, stream):
+    """The slab with FP32 fields (u and kappa rounded from the same seeded
+    values), K ghost planes per side, compensated passes of K steps with the
+    residual in registers (H3D_F32_CARRY), bands of K FP32 planes exchanged
+    in HBM once per pass; the interior pass shape from the autotuner at
+    depth K."""
+    planes = owned + 2 * K
+    g = h3d.pitched_grid(n, n, planes)
+    u, k = h3d.gen_seeded(g, gk_owned - K, n, n, n, 1, 20260101)
+    fields = []
+    for x in (u, u, k):
+        f = h3d.AlignedField(g, np.float32)
+        f.upload(x.astype(np.float32))
+        fields.append(f)
+    del u, k
+    da, db, dk = fields
+    lo, hi = h3d.DeviceArray(K * n * n * 4), h3d.DeviceArray(K * n * n * 4)
+    ms, fb = h3d.C.c_float(), h3d.C.c_int()
+    t = h3d.Tune()
+    h3d.call("h3d_autotune", 1, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 1, K, stream.ptr,
+             h3d.C.byref(t))
+    f32f = h3d.C.c_float(np.float32(factor))
+    args = (da.ptr, db.ptr, dk.ptr, None, None, None, h3d.C.byref(g), f32f, 1, K, lo.ptr,
+            hi.ptr, None, None)
+    h3d.call("h3d_run_slab_pass_f32", *args, 4 * K, h3d.C.byref(t), 1, stream.ptr,
+             h3d.C.byref(ms), h3d.C.byref(fb))
+    h3d.call("h3d_run_slab_pass_f32", *args, steps, h3d.C.byref(t), 1, stream.ptr,
+             h3d.C.byref(ms), h3d.C.byref(fb))
+    lups = (n - 2) * (n - 2) * owned * steps
+    passes = -(-steps // K)
+    # per pass: 2 band packs + 2 band unpacks of K FP32 planes, read and written
+    face_bytes = 4 * 2 * K * n * n * 4 * passes
+    gbs = (12.0 * lups / K + face_bytes) / (ms.value / 1e3) / 1e9
+    st = h3d.stats(db if fb.value else da, g, f32=True)
+    for f in fields + [lo, hi]:
+        f.close()
+    return {"glups": round(lups / (ms.value / 1e3) / 1e9, 3), "dtype": "f32 compensated "
+            "(residual in registers)", "ms_per_step": round(ms.value / steps, 5),
+            "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
+            "slab": [n, n, planes], "band_bytes": K * n * n * 4,
+            "tune": [t.rows, t.waves, t.zchunk, t.kblock], "nonfinite": float(st[3])}
 
 
 def leg_cfg4(bw, stream):

@@ -157,6 +157,16 @@ int h3d_pack_band_f64(double *buf, const double *u, const h3d_grid *g, int k,
                       int count, void *stream);
 int h3d_unpack_band_f64(double *u, const double *buf, const h3d_grid *g, int k,
                         int count, void *stream);
+/* The same for FP32 fields and for the FP16 residual field of
+ * H3D_F32_COMP16 (16-bit words). */
+int h3d_pack_band_f32(float *buf, const float *u, const h3d_grid *g, int k,
+                      int count, void *stream);
+int h3d_unpack_band_f32(float *u, const float *buf, const h3d_grid *g, int k,
+                        int count, void *stream);
+int h3d_pack_band_u16(uint16_t *buf, const uint16_t *u, const h3d_grid *g,
+                      int k, int count, void *stream);
+int h3d_unpack_band_u16(uint16_t *u, const uint16_t *buf, const h3d_grid *g,
+                        int k, int count, void *stream);
 
 /* One step of one z-slab (nz >= 4 planes: 2 ghosts + owned planes) in the
  * shape of the reference's middle devices (4-openmp-stream-omp-p2p.c:
@@ -208,6 +218,30 @@ int h3d_run_slab_pass_f64(double *u_a, double *u_b, const double *rhs,
                           const double *kappa, const h3d_grid *g,
                           double factor, int ghost, double *face_lo,
                           double *face_hi, int steps, const h3d_tune *tune,
+                          int graph, void *stream, float *ms, int *final_in_b);
+
+/* FP32 slabs with `ghost` (1..8) ghost planes per side: h3d_slab_pass_f64
+ * with FP32 fields and the FP32 pass (h3d_pass_f32_range, mode as there).
+ * In mode H3D_F32_COMP16 the residual field's bands move too, through
+ * cface_lo / cface_hi (ghost*nx*ny 16-bit words each; NULL in the other
+ * modes): a ghost plane's residual carries into the next pass as an owned
+ * plane's does.  Owned planes after a pass equal those planes of the same
+ * pass over the whole field, bit for bit.  No reference counterpart (the
+ * reference is FP64). */
+int h3d_slab_pass_f32(float *u_new, const float *u_old, const float *kappa_f32,
+                      const uint16_t *kappa_bf16, uint16_t *comp_new,
+                      const uint16_t *comp_old, const h3d_grid *g, float factor,
+                      int mode, int ghost, int nsteps, float *face_lo,
+                      float *face_hi, uint16_t *cface_lo, uint16_t *cface_hi,
+                      const h3d_tune *tune, void *stream);
+/* `steps` steps of such a slab as passes of `ghost` steps (the remainder
+ * one shorter pass) with u_a/u_b (and c_a/c_b) ping-pong, as
+ * h3d_run_slab_pass_f64; tune shapes the interior's full passes. */
+int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_f32,
+                          const uint16_t *kappa_bf16, uint16_t *c_a, uint16_t *c_b,
+                          const h3d_grid *g, float factor, int mode, int ghost,
+                          float *face_lo, float *face_hi, uint16_t *cface_lo,
+                          uint16_t *cface_hi, int steps, const h3d_tune *tune,
                           int graph, void *stream, float *ms, int *final_in_b);
 
 /* FP64 diagnostics over the whole array (ring included), deterministic:
@@ -271,6 +305,16 @@ int h3d_pass_f32(float *u_out, const float *u_in, const float *kappa_f32,
                  const uint16_t *kappa_bf16, uint16_t *comp_out,
                  const uint16_t *comp_in, const h3d_grid *g, float factor,
                  int nsteps, int mode, const h3d_tune *tune, void *stream);
+/* The same pass writing planes k_start..k_end only (1 <= k_start, k_end <=
+ * nz-2): those planes of u_out (and comp_out) receive those planes of
+ * nsteps steps applied to all of u_in, the bits of h3d_pass_f32; other
+ * planes are not written (as h3d_pass_f64 with a plane range).  The
+ * exchange unit of FP32 slabs with deep ghosts (h3d_slab_pass_f32). */
+int h3d_pass_f32_range(float *u_out, const float *u_in, const float *kappa_f32,
+                       const uint16_t *kappa_bf16, uint16_t *comp_out,
+                       const uint16_t *comp_in, const h3d_grid *g, float factor,
+                       int nsteps, int mode, int k_start, int k_end,
+                       const h3d_tune *tune, void *stream);
 
 /* `steps` FP32 updates as passes of tune->kblock steps (remainder last),
  * ping-ponging u_a/u_b (and c_a/c_b in COMP16 mode).  graph != 0 captures

@@ -71,8 +71,8 @@ h3d_pass_f32_kernel(float *__restrict__ uout, const float *__restrict__ uin,
                     const unsigned short *__restrict__ kb16,
                     unsigned short *__restrict__ cout,
                     const unsigned short *__restrict__ cin, int nx, int ny,
-                    int nz, int sy, long long sz, float factor, int zchunk,
-                    int tiles_x, int tiles_y) {
+                    int nz, int sy, long long sz, float factor, int ks, int ke,
+                    int zchunk, int tiles_x, int tiles_y) {
   extern __shared__ float lds[];
   // one (tile, z chunk) per workgroup, z chunk slowest: the tiles of a
   // chunk start together and march in lockstep, so the halo columns and
@@ -81,10 +81,10 @@ h3d_pass_f32_kernel(float *__restrict__ uout, const float *__restrict__ uin,
   const int tx = (int)(b % (unsigned)tiles_x);
   const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
   const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
-  const int zb = 1 + tz * zchunk;
+  const int zb = ks + tz * zchunk;
   pass_f32_tile<KB, RY, W, MODE, BF16>(lds, uout, uin, kf, kb16, cout, cin, nx,
                                        ny, nz, sy, sz, factor, zb,
-                                       min(nz - 2, zb + zchunk - 1), tx, ty);
+                                       min(ke, zb + zchunk - 1), tx, ty);
 }
 
 // Arguments of one FP32 pass launch.  With `occ` set, nothing is launched:
@@ -99,6 +99,7 @@ struct F32Launch {
   const uint16_t *ci;
   const h3d_grid *g;
   float factor;
+  int ks, ke;  // output planes
   int zchunk;
   hipStream_t s;
   bool bf16;  // kappa as BF16 bits (kb) rather than FP32 (kf)
@@ -134,12 +135,12 @@ int launch_one(const F32Launch &a) {
     return H3D_OK;
   }
   const long long nb =
-      (long long)tiles_x * tiles_y * ((g->nz - 2 + a.zchunk - 1) / a.zchunk);
+      (long long)tiles_x * tiles_y * ((a.ke - a.ks + 1 + a.zchunk - 1) / a.zchunk);
   if (nb > 0x7fffffffLL) return H3D_ELIMIT;
   hipLaunchKernelGGL(fn, dim3((unsigned)nb), dim3(64 * W), lds, a.s, a.uo, a.ui, a.kf,
                      (const unsigned short *)a.kb, (unsigned short *)a.co,
                      (const unsigned short *)a.ci, g->nx, g->ny, g->nz, g->sy,
-                     g->sz, a.factor, a.zchunk, tiles_x, tiles_y);
+                     g->sz, a.factor, a.ks, a.ke, a.zchunk, tiles_x, tiles_y);
   return h3d_launch_status();
 }
 
@@ -190,60 +191,53 @@ int launch_pass(int kb, int ry, int waves, const F32Launch &a, int mode) {
 }
 
 // Single-step passes (KB = 1): wave-independent tiles of 64 x (waves * RY)
-// cells marching z, one (tile, z chunk) per workgroup, XCD-aware order.
+// cells marching z over planes ks..ke, one (tile, z chunk) per workgroup,
+// XCD-aware order.
 template <int RY, int MODE, bool BF16>
 __global__ void __launch_bounds__(512)
 h3d_march_f32(float *__restrict__ uout, const float *__restrict__ uin,
               const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
               unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
-              int nx, int ny, int nz, int sy, long long sz, float factor,
-              int zchunk, int tiles_x, int tiles_y) {
+              int nx, int ny, int nz, int sy, long long sz, float factor, int ks,
+              int ke, int zchunk, int tiles_x, int tiles_y) {
   const unsigned b = h3d_xcd_tile(blockIdx.x, gridDim.x);
   const int tx = (int)(b % (unsigned)tiles_x);
   const int ty = (int)((b / (unsigned)tiles_x) % (unsigned)tiles_y);
   const int tz = (int)(b / (unsigned)tiles_x / (unsigned)tiles_y);
   march_f32_tile<RY, MODE, BF16>(uout, uin, kf, kb16, cout, cin, nx, ny, nz, sy, sz,
-                                 factor, zchunk, tx, ty, tz);
+                                 factor, ks, ke, zchunk, tx, ty, tz);
 }
 
 template <int RY, int MODE, bool BF16>
-int launch_march_one(int waves, float *uo, const float *ui, const float *kf,
-                     const uint16_t *kb, uint16_t *co, const uint16_t *ci,
-                     const h3d_grid *g, float factor, int zchunk, hipStream_t s) {
+int launch_march_one(int waves, const F32Launch &a) {
+  const h3d_grid *g = a.g;
   const int tiles_x = (g->nx - 2 + 63) / 64;
   const int tiles_y = (g->ny - 2 + RY * waves - 1) / (RY * waves);
   const long long nb =
-      (long long)tiles_x * tiles_y * ((g->nz - 2 + zchunk - 1) / zchunk);
+      (long long)tiles_x * tiles_y * ((a.ke - a.ks + 1 + a.zchunk - 1) / a.zchunk);
   if (nb > 0x7fffffffLL) return H3D_ELIMIT;
   hipLaunchKernelGGL((h3d_march_f32<RY, MODE, BF16>), dim3((unsigned)nb), dim3(64 * waves),
-                     0, s, uo, ui, kf, (const unsigned short *)kb, (unsigned short *)co,
-                     (const unsigned short *)ci, g->nx, g->ny, g->nz, g->sy, g->sz, factor,
-                     zchunk, tiles_x, tiles_y);
+                     0, a.s, a.uo, a.ui, a.kf, (const unsigned short *)a.kb,
+                     (unsigned short *)a.co, (const unsigned short *)a.ci, g->nx, g->ny,
+                     g->nz, g->sy, g->sz, a.factor, a.ks, a.ke, a.zchunk, tiles_x, tiles_y);
   return h3d_launch_status();
 }
 
 template <int RY>
-int launch_march_ry(int waves, float *uo, const float *ui, const float *kf,
-                    const uint16_t *kb, uint16_t *co, const uint16_t *ci,
-                    const h3d_grid *g, float factor, int mode, int zchunk,
-                    hipStream_t s) {
-  const bool bf = kb != nullptr;
+int launch_march_ry(int waves, const F32Launch &a, int mode) {
 #define H3D_M(M) \
-  return bf ? launch_march_one<RY, M, true>(waves, uo, ui, kf, kb, co, ci, g, factor, zchunk, s) \
-            : launch_march_one<RY, M, false>(waves, uo, ui, kf, kb, co, ci, g, factor, zchunk, s)
+  return a.bf16 ? launch_march_one<RY, M, true>(waves, a) : launch_march_one<RY, M, false>(waves, a)
   if (mode == 0) H3D_M(0);
   if (mode == 1) H3D_M(1);
   H3D_M(2);
 #undef H3D_M
 }
 
-int launch_march(int ry, int waves, float *uo, const float *ui, const float *kf,
-                 const uint16_t *kb, uint16_t *co, const uint16_t *ci,
-                 const h3d_grid *g, float factor, int mode, int zchunk, hipStream_t s) {
+int launch_march(int ry, int waves, const F32Launch &a, int mode) {
   if (waves < 1 || waves > 8) return H3D_EARG;
-  if (ry == 1) return launch_march_ry<1>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
-  if (ry == 2) return launch_march_ry<2>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
-  if (ry == 4) return launch_march_ry<4>(waves, uo, ui, kf, kb, co, ci, g, factor, mode, zchunk, s);
+  if (ry == 1) return launch_march_ry<1>(waves, a, mode);
+  if (ry == 2) return launch_march_ry<2>(waves, a, mode);
+  if (ry == 4) return launch_march_ry<4>(waves, a, mode);
   return H3D_EARG;
 }
 
@@ -304,12 +298,14 @@ int h3d_pass_f32_zchunks(const h3d_grid *g, int kblock, int rows, int waves, int
   return h3d_round_zchunks(g->nz - 2, kblock, tiles, occ, zc, n, got);
 }
 
-extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
-                            const float *kappa_f32, const uint16_t *kappa_bf16,
-                            uint16_t *comp_out, const uint16_t *comp_in,
-                            const h3d_grid *g, float factor, int nsteps,
-                            int mode, const h3d_tune *tune, void *stream) {
+extern "C" int h3d_pass_f32_range(float *u_out, const float *u_in,
+                                  const float *kappa_f32, const uint16_t *kappa_bf16,
+                                  uint16_t *comp_out, const uint16_t *comp_in,
+                                  const h3d_grid *g, float factor, int nsteps, int mode,
+                                  int k_start, int k_end, const h3d_tune *tune,
+                                  void *stream) {
   if (!u_out || !u_in || u_out == u_in || !h3d_grid_ok(g)) return H3D_EARG;
+  if (k_start < 1 || k_end > g->nz - 2) return H3D_EARG;
   if ((kappa_f32 == nullptr) == (kappa_bf16 == nullptr)) return H3D_EARG;
   if (mode < 0 || mode > 2) return H3D_EARG;
   if (mode == 2 && (!comp_out || !comp_in || comp_out == comp_in)) return H3D_EARG;
@@ -344,13 +340,21 @@ extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
     }
   }
   if (t.zchunk < 1) return H3D_EARG;
-  hipStream_t s = h3d_stream(stream);
-  if (nsteps == 1)
-    return launch_march(t.rows, t.waves, u_out, u_in, kappa_f32, kappa_bf16, comp_out,
-                        comp_in, g, factor, mode, t.zchunk, s);
-  F32Launch a{u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor, t.zchunk,
-              s, bf != 0, nullptr, nullptr};
+  if (k_end < k_start) return H3D_OK;  // empty range: nothing to do
+  F32Launch a{u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor, k_start,
+              k_end, t.zchunk, h3d_stream(stream), bf != 0, nullptr, nullptr};
+  if (nsteps == 1) return launch_march(t.rows, t.waves, a, mode);
   return launch_pass(nsteps, t.rows, t.waves, a, mode);
+}
+
+extern "C" int h3d_pass_f32(float *u_out, const float *u_in,
+                            const float *kappa_f32, const uint16_t *kappa_bf16,
+                            uint16_t *comp_out, const uint16_t *comp_in,
+                            const h3d_grid *g, float factor, int nsteps,
+                            int mode, const h3d_tune *tune, void *stream) {
+  if (!h3d_grid_ok(g)) return H3D_EARG;
+  return h3d_pass_f32_range(u_out, u_in, kappa_f32, kappa_bf16, comp_out, comp_in, g, factor,
+                            nsteps, mode, 1, g->nz - 2, tune, stream);
 }
 
 extern "C" int h3d_run_f32(float *u_a, float *u_b, const float *kappa_f32,

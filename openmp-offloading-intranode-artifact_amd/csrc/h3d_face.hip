@@ -104,3 +104,19 @@ extern "C" int h3d_unpack_band_f64(double *u, const double *buf, const h3d_grid 
                                    int k, int count, void *stream) {
   return plane_copy<double>(u, buf, g, k, count, false, h3d_stream(stream));
 }
+extern "C" int h3d_pack_band_f32(float *buf, const float *u, const h3d_grid *g, int k,
+                                 int count, void *stream) {
+  return plane_copy<float>(buf, u, g, k, count, true, h3d_stream(stream));
+}
+extern "C" int h3d_unpack_band_f32(float *u, const float *buf, const h3d_grid *g, int k,
+                                   int count, void *stream) {
+  return plane_copy<float>(u, buf, g, k, count, false, h3d_stream(stream));
+}
+extern "C" int h3d_pack_band_u16(uint16_t *buf, const uint16_t *u, const h3d_grid *g, int k,
+                                 int count, void *stream) {
+  return plane_copy<uint16_t>(buf, u, g, k, count, true, h3d_stream(stream));
+}
+extern "C" int h3d_unpack_band_u16(uint16_t *u, const uint16_t *buf, const h3d_grid *g,
+                                   int k, int count, void *stream) {
+  return plane_copy<uint16_t>(u, buf, g, k, count, false, h3d_stream(stream));
+}

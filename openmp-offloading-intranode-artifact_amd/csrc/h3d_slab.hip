@@ -35,33 +35,28 @@ namespace {
 // the planes within ns of its output planes, all of which the slab holds,
 // so its owned planes are exactly ns steps of the update; ghost values the
 // pass computes on the way are not kept (the exchange overwrites them).
-int slab_pass(double *un, const double *uo, const double *rhs,
-              const double *kap, const h3d_grid *g, double factor, int gh,
-              int ns, double *face_lo, double *face_hi, const h3d_tune *tune,
-              hipStream_t s, hipStream_t s2, hipEvent_t fork, hipEvent_t join) {
-  const int nz = g->nz;
+// pass(ks, ke, interior, stream) runs the pass over planes ks..ke (the
+// interior range with the caller's shape); exchange(stream) moves the
+// bands.
+template <typename Pass, typename Exchange>
+int slab_pass(int nz, int gh, Pass pass, Exchange exchange, hipStream_t s, hipStream_t s2,
+              hipEvent_t fork, hipEvent_t join) {
   const int lo_end = 2 * gh - 1;            // lower band gh .. 2gh-1
   const int hi_beg = max(nz - 2 * gh, 2 * gh);  // upper band .. nz-1-gh
   const int in_beg = 2 * gh, in_end = nz - 2 * gh - 1;
   const bool interior = in_end >= in_beg;
-  auto pass = [&](int ks, int ke, const h3d_tune *t, hipStream_t st) {
-    return h3d_pass_f64(un, uo, rhs, kap, g, factor, ns, ks, ke, t, st);
-  };
   if (interior && s2) {
     H3D_TRY(hipEventRecord(fork, s));
     H3D_TRY(hipStreamWaitEvent(s2, fork, 0));
-    int rc = pass(in_beg, in_end, tune, s2);
+    int rc = pass(in_beg, in_end, true, s2);
     if (rc != H3D_OK) return rc;
   }
-  int rc = pass(gh, lo_end, nullptr, s);
-  if (rc == H3D_OK && hi_beg <= nz - 1 - gh) rc = pass(hi_beg, nz - 1 - gh, nullptr, s);
-  if (rc == H3D_OK) rc = h3d_pack_band_f64(face_lo, un, g, gh, gh, s);
-  if (rc == H3D_OK) rc = h3d_pack_band_f64(face_hi, un, g, nz - 2 * gh, gh, s);
-  if (rc == H3D_OK) rc = h3d_unpack_band_f64(un, face_hi, g, 0, gh, s);
-  if (rc == H3D_OK) rc = h3d_unpack_band_f64(un, face_lo, g, nz - gh, gh, s);
+  int rc = pass(gh, lo_end, false, s);
+  if (rc == H3D_OK && hi_beg <= nz - 1 - gh) rc = pass(hi_beg, nz - 1 - gh, false, s);
+  if (rc == H3D_OK) rc = exchange(s);
   if (rc != H3D_OK) return rc;
   if (interior && !s2) {
-    rc = pass(in_beg, in_end, tune, s);
+    rc = pass(in_beg, in_end, true, s);
     if (rc != H3D_OK) return rc;
   }
   if (interior && s2) {
@@ -71,20 +66,77 @@ int slab_pass(double *un, const double *uo, const double *rhs,
   return H3D_OK;
 }
 
-// Ghost depth 1..4 (the fused pass's depths), at least gh owned planes
-// (the bands must be owned), and one ghost slab at least 4 planes deep.
-bool slab_args_ok(const void *un, const void *uo, const void *kap,
-                  const h3d_grid *g, int gh, const void *lo, const void *hi) {
-  return un && uo && kap && lo && hi && un != uo && lo != hi && h3d_grid_ok(g) &&
-         gh >= 1 && gh <= 4 && g->nz >= 3 * gh && g->nz >= 4;
+// The band exchange of one field: owned planes nz-2gh .. nz-1-gh into
+// ghosts 0 .. gh-1 and owned planes gh .. 2gh-1 into ghosts nz-gh .. nz-1,
+// through the device buffers lo and hi of gh planes.
+template <typename T, typename Pack, typename Unpack>
+int exchange_bands(T *u, const h3d_grid *g, int gh, T *lo, T *hi, Pack pack, Unpack unpack,
+                   hipStream_t s) {
+  const int nz = g->nz;
+  int rc = pack(lo, u, g, gh, gh, s);
+  if (rc == H3D_OK) rc = pack(hi, u, g, nz - 2 * gh, gh, s);
+  if (rc == H3D_OK) rc = unpack(u, hi, g, 0, gh, s);
+  if (rc == H3D_OK) rc = unpack(u, lo, g, nz - gh, gh, s);
+  return rc;
 }
 
-int run_slab(double *u_a, double *u_b, const double *rhs, const double *kappa,
-             const h3d_grid *g, double factor, int gh, double *face_lo,
-             double *face_hi, int steps, const h3d_tune *tune, int graph,
-             void *stream, float *ms, int *final_in_b) {
-  if (!slab_args_ok(u_a, u_b, kappa, g, gh, face_lo, face_hi) || steps < 0)
-    return H3D_EARG;
+int slab_pass_f64(double *un, const double *uo, const double *rhs, const double *kap,
+                  const h3d_grid *g, double factor, int gh, int ns, double *face_lo,
+                  double *face_hi, const h3d_tune *tune, hipStream_t s, hipStream_t s2,
+                  hipEvent_t fork, hipEvent_t join) {
+  auto pass = [&](int ks, int ke, bool interior, hipStream_t st) {
+    return h3d_pass_f64(un, uo, rhs, kap, g, factor, ns, ks, ke, interior ? tune : nullptr,
+                        st);
+  };
+  auto exchange = [&](hipStream_t st) {
+    return exchange_bands(un, g, gh, face_lo, face_hi, h3d_pack_band_f64,
+                          h3d_unpack_band_f64, st);
+  };
+  return slab_pass(g->nz, gh, pass, exchange, s, s2, fork, join);
+}
+
+int slab_pass_f32(float *un, const float *uo, const float *kf, const uint16_t *kb16,
+                  uint16_t *cn, const uint16_t *co, const h3d_grid *g, float factor, int mode,
+                  int gh, int ns, float *face_lo, float *face_hi, uint16_t *cface_lo,
+                  uint16_t *cface_hi, const h3d_tune *tune, hipStream_t s, hipStream_t s2,
+                  hipEvent_t fork, hipEvent_t join) {
+  // the caller's shape for full passes of the interior; the bands and a
+  // shorter remainder pass take their depth's default shape (FP32 shapes
+  // differ between depths)
+  const h3d_tune *ti = (tune && ns == gh) ? tune : nullptr;
+  auto pass = [&](int ks, int ke, bool interior, hipStream_t st) {
+    return h3d_pass_f32_range(un, uo, kf, kb16, cn, co, g, factor, ns, mode, ks, ke,
+                              interior ? ti : nullptr, st);
+  };
+  auto exchange = [&](hipStream_t st) {
+    int rc = exchange_bands(un, g, gh, face_lo, face_hi, h3d_pack_band_f32,
+                            h3d_unpack_band_f32, st);
+    // the FP16 residual of the ghost planes carries into the next pass
+    if (rc == H3D_OK && mode == H3D_F32_COMP16)
+      rc = exchange_bands(cn, g, gh, cface_lo, cface_hi, h3d_pack_band_u16,
+                          h3d_unpack_band_u16, st);
+    return rc;
+  };
+  return slab_pass(g->nz, gh, pass, exchange, s, s2, fork, join);
+}
+
+// Ghost depth 1..max_gh (the fused pass's depths), at least gh owned planes
+// (the bands must be owned), and one ghost slab at least 4 planes deep.
+bool slab_args_ok(const void *un, const void *uo, const void *kap,
+                  const h3d_grid *g, int gh, const void *lo, const void *hi, int max_gh = 4) {
+  return un && uo && kap && lo && hi && un != uo && lo != hi && h3d_grid_ok(g) &&
+         gh >= 1 && gh <= max_gh && g->nz >= 3 * gh && g->nz >= 4;
+}
+
+// `steps` steps as passes of gh steps, the remainder as one shorter pass,
+// the buffers swapping once per pass: pass(even, ns, s, s2, fork, join)
+// issues pass p (even: p is even, reading the a buffers) of ns steps.  A
+// second stream carries the interior of each pass; graph != 0 replays a
+// captured pair of full passes.
+template <typename Pass>
+int run_slab(int gh, int steps, int graph, void *stream, float *ms, int *final_in_b,
+             Pass pass) {
+  if (steps < 0) return H3D_EARG;
   hipStream_t s = h3d_stream(stream);
   if (!s) return H3D_EARG;  // the overlap needs a created stream to fork from
   hipStream_t s2 = nullptr;
@@ -92,8 +144,6 @@ int run_slab(double *u_a, double *u_b, const double *rhs, const double *kappa,
   hipGraph_t gr = nullptr;
   hipGraphExec_t ge = nullptr;
   int rc = H3D_OK;
-  // passes of gh steps, the remainder as one shorter pass; the buffers
-  // swap once per pass
   const int full = steps / gh, rem = steps % gh;
   const int npass = full + (rem ? 1 : 0);
   hipError_t he = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
@@ -106,8 +156,8 @@ int run_slab(double *u_a, double *u_b, const double *rhs, const double *kappa,
     he = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
     if (he != hipSuccess) rc = (int)he;
     if (rc == H3D_OK) {
-      int r1 = slab_pass(u_b, u_a, rhs, kappa, g, factor, gh, gh, face_lo, face_hi, tune, s, s2, fork, join);
-      int r2 = r1 == H3D_OK ? slab_pass(u_a, u_b, rhs, kappa, g, factor, gh, gh, face_lo, face_hi, tune, s, s2, fork, join) : r1;
+      int r1 = pass(true, gh, s, s2, fork, join);
+      int r2 = r1 == H3D_OK ? pass(false, gh, s, s2, fork, join) : r1;
       he = hipStreamEndCapture(s, &gr);
       if (r2 != H3D_OK) rc = r2;
       else if (he != hipSuccess) rc = (int)he;
@@ -128,11 +178,8 @@ int run_slab(double *u_a, double *u_b, const double *rhs, const double *kappa,
       if (he != hipSuccess) { rc = (int)he; break; }
     }
   }
-  for (; rc == H3D_OK && done < npass; ++done) {
-    const bool even = done % 2 == 0;
-    rc = slab_pass(even ? u_b : u_a, even ? u_a : u_b, rhs, kappa, g, factor, gh,
-                   done < full ? gh : rem, face_lo, face_hi, tune, s, s2, fork, join);
-  }
+  for (; rc == H3D_OK && done < npass; ++done)
+    rc = pass(done % 2 == 0, done < full ? gh : rem, s, s2, fork, join);
   if (rc == H3D_OK) {
     float t = 0.f;
     he = hipEventRecord(e1, s);
@@ -157,7 +204,34 @@ int run_slab(double *u_a, double *u_b, const double *rhs, const double *kappa,
   return rc;
 }
 
+int run_slab_f64(double *u_a, double *u_b, const double *rhs, const double *kappa,
+                 const h3d_grid *g, double factor, int gh, double *face_lo,
+                 double *face_hi, int steps, const h3d_tune *tune, int graph,
+                 void *stream, float *ms, int *final_in_b) {
+  if (!slab_args_ok(u_a, u_b, kappa, g, gh, face_lo, face_hi)) return H3D_EARG;
+  return run_slab(gh, steps, graph, stream, ms, final_in_b,
+                  [&](bool even, int ns, hipStream_t s, hipStream_t s2, hipEvent_t fork,
+                      hipEvent_t join) {
+                    return slab_pass_f64(even ? u_b : u_a, even ? u_a : u_b, rhs, kappa, g,
+                                         factor, gh, ns, face_lo, face_hi, tune, s, s2, fork,
+                                         join);
+                  });
+}
+
+bool slab_f32_args_ok(const float *un, const float *uo, const float *kf, const uint16_t *kb16,
+                      const uint16_t *cn, const uint16_t *co, const h3d_grid *g, int mode,
+                      int gh, const void *lo, const void *hi, const void *clo,
+                      const void *chi) {
+  if ((kf == nullptr) == (kb16 == nullptr)) return false;
+  if (mode < 0 || mode > 2) return false;
+  if (mode == H3D_F32_COMP16 &&
+      (!cn || !co || cn == co || !clo || !chi || clo == chi))
+    return false;
+  return slab_args_ok(un, uo, kf ? (const void *)kf : (const void *)kb16, g, gh, lo, hi, 8);
+}
+
 }  // namespace
+
 
 extern "C" int h3d_slab_step_f64(double *u_new, const double *u_old,
                                  const double *rhs, const double *kappa,
@@ -165,8 +239,8 @@ extern "C" int h3d_slab_step_f64(double *u_new, const double *u_old,
                                  double *face_lo, double *face_hi,
                                  const h3d_tune *tune, void *stream) {
   if (!slab_args_ok(u_new, u_old, kappa, g, 1, face_lo, face_hi)) return H3D_EARG;
-  return slab_pass(u_new, u_old, rhs, kappa, g, factor, 1, 1, face_lo, face_hi, tune,
-                   h3d_stream(stream), nullptr, nullptr, nullptr);
+  return slab_pass_f64(u_new, u_old, rhs, kappa, g, factor, 1, 1, face_lo, face_hi, tune,
+                       h3d_stream(stream), nullptr, nullptr, nullptr);
 }
 
 extern "C" int h3d_run_slab_f64(double *u_a, double *u_b, const double *rhs,
@@ -175,8 +249,8 @@ extern "C" int h3d_run_slab_f64(double *u_a, double *u_b, const double *rhs,
                                 double *face_hi, int steps,
                                 const h3d_tune *tune, int graph, void *stream,
                                 float *ms, int *final_in_b) {
-  return run_slab(u_a, u_b, rhs, kappa, g, factor, 1, face_lo, face_hi, steps, tune,
-                  graph, stream, ms, final_in_b);
+  return run_slab_f64(u_a, u_b, rhs, kappa, g, factor, 1, face_lo, face_hi, steps, tune,
+                      graph, stream, ms, final_in_b);
 }
 
 extern "C" int h3d_slab_pass_f64(double *u_new, const double *u_old,
@@ -186,8 +260,8 @@ extern "C" int h3d_slab_pass_f64(double *u_new, const double *u_old,
                                  const h3d_tune *tune, void *stream) {
   if (!slab_args_ok(u_new, u_old, kappa, g, ghost, face_lo, face_hi)) return H3D_EARG;
   if (nsteps < 1 || nsteps > ghost) return H3D_EARG;
-  return slab_pass(u_new, u_old, rhs, kappa, g, factor, ghost, nsteps, face_lo, face_hi,
-                   tune, h3d_stream(stream), nullptr, nullptr, nullptr);
+  return slab_pass_f64(u_new, u_old, rhs, kappa, g, factor, ghost, nsteps, face_lo, face_hi,
+                       tune, h3d_stream(stream), nullptr, nullptr, nullptr);
 }
 
 extern "C" int h3d_run_slab_pass_f64(double *u_a, double *u_b, const double *rhs,
@@ -196,6 +270,40 @@ extern "C" int h3d_run_slab_pass_f64(double *u_a, double *u_b, const double *rhs
                                      double *face_hi, int steps,
                                      const h3d_tune *tune, int graph,
                                      void *stream, float *ms, int *final_in_b) {
-  return run_slab(u_a, u_b, rhs, kappa, g, factor, ghost, face_lo, face_hi, steps, tune,
-                  graph, stream, ms, final_in_b);
+  return run_slab_f64(u_a, u_b, rhs, kappa, g, factor, ghost, face_lo, face_hi, steps, tune,
+                      graph, stream, ms, final_in_b);
+}
+
+extern "C" int h3d_slab_pass_f32(float *u_new, const float *u_old, const float *kappa_f32,
+                                 const uint16_t *kappa_bf16, uint16_t *comp_new,
+                                 const uint16_t *comp_old, const h3d_grid *g, float factor,
+                                 int mode, int ghost, int nsteps, float *face_lo,
+                                 float *face_hi, uint16_t *cface_lo, uint16_t *cface_hi,
+                                 const h3d_tune *tune, void *stream) {
+  if (!slab_f32_args_ok(u_new, u_old, kappa_f32, kappa_bf16, comp_new, comp_old, g, mode,
+                        ghost, face_lo, face_hi, cface_lo, cface_hi))
+    return H3D_EARG;
+  if (nsteps < 1 || nsteps > ghost) return H3D_EARG;
+  return slab_pass_f32(u_new, u_old, kappa_f32, kappa_bf16, comp_new, comp_old, g, factor,
+                       mode, ghost, nsteps, face_lo, face_hi, cface_lo, cface_hi, tune,
+                       h3d_stream(stream), nullptr, nullptr, nullptr);
+}
+
+extern "C" int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_f32,
+                                     const uint16_t *kappa_bf16, uint16_t *c_a, uint16_t *c_b,
+                                     const h3d_grid *g, float factor, int mode, int ghost,
+                                     float *face_lo, float *face_hi, uint16_t *cface_lo,
+                                     uint16_t *cface_hi, int steps, const h3d_tune *tune,
+                                     int graph, void *stream, float *ms, int *final_in_b) {
+  if (!slab_f32_args_ok(u_a, u_b, kappa_f32, kappa_bf16, c_a, c_b, g, mode, ghost, face_lo,
+                        face_hi, cface_lo, cface_hi))
+    return H3D_EARG;
+  return run_slab(ghost, steps, graph, stream, ms, final_in_b,
+                  [&](bool even, int ns, hipStream_t s, hipStream_t s2, hipEvent_t fork,
+                      hipEvent_t join) {
+                    return slab_pass_f32(even ? u_b : u_a, even ? u_a : u_b, kappa_f32,
+                                         kappa_bf16, even ? c_b : c_a, even ? c_a : c_b, g,
+                                         factor, mode, ghost, ns, face_lo, face_hi, cface_lo,
+                                         cface_hi, tune, s, s2, fork, join);
+                  });
 }

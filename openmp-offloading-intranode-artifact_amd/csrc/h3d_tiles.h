@@ -790,7 +790,8 @@ __device__ __forceinline__ void pass_f64_tile(
   for (int p = 0; p < groups; ++p) st.template group<0>(it0 + U * p);
 }
 
-// One tile (tx, ty, tz) of a single-step FP32 pass (KB = 1), built like the
+// One tile (tx, ty, tz) of a single-step FP32 pass (KB = 1) over planes
+// k_start..k_end (z chunk tz), built like the
 // FP64 tile: waves x 64 lanes, RY rows per lane, independent wavefronts (no
 // LDS, no barrier), z-neighbours in registers, x-neighbours through DPP with
 // lanes 0 and 63 loading the x halo, y halo rows loaded per wave.  All
@@ -806,8 +807,8 @@ __device__ __forceinline__ void march_f32_tile(
     float *__restrict__ uout, const float *__restrict__ uin,
     const float *__restrict__ kf, const unsigned short *__restrict__ kb16,
     unsigned short *__restrict__ cout, const unsigned short *__restrict__ cin,
-    int nx, int ny, int nz, int sy, long long sz, float factor, int zchunk,
-    int tx, int ty, int tz) {
+    int nx, int ny, int nz, int sy, long long sz, float factor, int k_start,
+    int k_end, int zchunk, int tx, int ty, int tz) {
 #pragma clang fp contract(off)
   constexpr unsigned OOB = 0x80000000u;  // past every plane (<= 2^28 elements)
   const int lane = threadIdx.x & 63;
@@ -816,9 +817,9 @@ __device__ __forceinline__ void march_f32_tile(
   const int i = 1 + tx * 64 + lane;
   const int j0 = 1 + (ty * waves + wave) * RY;
   if (j0 > ny - 2) return;  // wave-uniform; no barriers in this function
-  const int kb = 1 + tz * zchunk;
-  if (kb > nz - 2) return;
-  const int ke = min(nz - 2, kb + zchunk - 1);
+  const int kb = k_start + tz * zchunk;
+  if (kb > k_end) return;
+  const int ke = min(k_end, kb + zchunk - 1);
   const unsigned b4 = (unsigned)sz * 4u, b2 = (unsigned)sz * 2u;
 
   const bool lane_ld = i <= nx - 1;

@@ -73,6 +73,15 @@ SIGNATURES = {
                                    _P, C.POINTER(C.c_float), C.POINTER(_I)]),
     "h3d_pack_band_f64": (_I, [_P, _P, _GP, _I, _I, _P]),
     "h3d_unpack_band_f64": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_pack_band_f32": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_unpack_band_f32": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_pack_band_u16": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_unpack_band_u16": (_I, [_P, _P, _GP, _I, _I, _P]),
+    "h3d_slab_pass_f32": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _I, _P, _P,
+                               _P, _P, _TP, _P]),
+    "h3d_run_slab_pass_f32": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _P, _P,
+                                   _P, _P, _I, _TP, _I, _P, C.POINTER(C.c_float),
+                                   C.POINTER(C.c_int)]),
     "h3d_pack_face_f64": (_I, [_P, _P, _GP, _I, _P]),
     "h3d_unpack_face_f64": (_I, [_P, _P, _GP, _I, _P]),
     "h3d_pack_face_f32": (_I, [_P, _P, _GP, _I, _P]),
@@ -84,6 +93,8 @@ SIGNATURES = {
     "h3d_compare_f32": (_I, [_P, _P, _GP, _D, _D, _P, _P]),
     "h3d_face_sum_max_f64": (_I, [_P, _GP, _P, _P]),
     "h3d_pass_f32": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _TP, _P]),
+    "h3d_pass_f32_range": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _I, _I, _TP,
+                                _P]),
     "h3d_run_f32": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _TP,
                          _I, _P, C.POINTER(C.c_float), C.POINTER(_I)]),
     "h3d_autotune": (_I, [_I, _P, _P, _P, _I, _GP, _I, _I, _P, _TP]),
