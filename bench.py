@@ -492,11 +492,20 @@ def slab_deep_f32(n, owned, gk_owned, K, factor, steps, bw, stream):
     st = h3d.stats(db if fb.value else da, g, f32=True)
     for f in fields + [lo, hi]:
         f.close()
+    # the interior pass (planes 2K..nz-2K-1) dominates a pass: u and kappa
+    # read, u written, 4 B each per cell
+    key = pass_key("h3d_pass_f32_kernel", K, t.rows, t.waves, t.zchunk, n - 2, ", 1, false")
+    key = key.rsplit("|", 2)[0] + "|" + str(
+        -(-(n - 2) // (64 - 2 * K)) * -(-(n - 2) // (t.rows * t.waves - 2 * K))
+        * -(-(planes - 4 * K) // t.zchunk) * 64 * t.waves) + "|" + str(64 * t.waves)
+    alg = 12.0 * (n - 2) ** 2 * (planes - 4 * K)
+    note_launch(f"slab_fp32c_ghost{K}_interior", key, alg)
     return {"glups": round(lups / (ms.value / 1e3) / 1e9, 3), "dtype": "f32 compensated "
             "(residual in registers)", "ms_per_step": round(ms.value / steps, 5),
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
             "slab": [n, n, planes], "band_bytes": K * n * n * 4,
-            "tune": [t.rows, t.waves, t.zchunk, t.kblock], "nonfinite": float(st[3])}
+            "tune": [t.rows, t.waves, t.zchunk, t.kblock], "nonfinite": float(st[3]),
+            "launch": key, "traffic": traffic_ratio(key, alg)}
 
 
 def leg_cfg4(bw, stream):
