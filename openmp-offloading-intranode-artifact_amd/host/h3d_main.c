@@ -34,8 +34,12 @@
  *                      two ghost planes refreshed every step by in-HBM face
  *                      pack/unpack from the slab's own edge planes; with
  *                      --kblock K, K ghost planes per side refreshed by a
- *                      band of K planes after every K-step fused pass
- *   --check            FP32 runs: also run the FP64 kernel on the same
+ *                      band of K planes after every K-step fused pass.
+ *                      FP32 precisions (seeded mode) run passes of K steps
+ *                      (1 without --kblock) with bands of K FP32 planes,
+ *                      and of the FP16 residual too in f32c16
+ *   --check            FP32 runs: also run the FP64 kernel (an FP64 slab
+ *                      with the same exchange, for --slab) on the same
  *                      inputs and report rel-L2 / max errors against it
  *   --device D         HIP device (default 0)
  * Exit status: 0 on success, 1 on any error (message on stderr).
@@ -168,10 +172,6 @@ static opts parse(int argc, char **argv)
   }
   if (o.bf16 && o.prec == 0) {
     fprintf(stderr, "ERROR: --kappa-bf16 needs an FP32 precision\n");
-    exit(1);
-  }
-  if (o.slab_parts && o.prec != 0) {
-    fprintf(stderr, "ERROR: --slab runs the FP64 kernel only\n");
     exit(1);
   }
   if (o.slab_parts && o.tune.kblock > 4) {
@@ -445,11 +445,17 @@ int main(int argc, char **argv)
            tune.waves, tune.zchunk, tune.kblock);
   }
 
-  /* slab: face buffers for the in-HBM exchange */
+  /* slab: face (band) buffers for the in-HBM exchange, sized for FP64;
+   * the FP16 residual's bands in f32c16 */
   double *d_face_lo = NULL, *d_face_hi = NULL;
+  uint16_t *d_cface_lo = NULL, *d_cface_hi = NULL;
   if (o.slab_parts) {
     CHECK(h3d_target_alloc((void **)&d_face_lo, (size_t)ghost * n * n * 8, o.device));
     CHECK(h3d_target_alloc((void **)&d_face_hi, (size_t)ghost * n * n * 8, o.device));
+    if (o.prec == 3) {
+      CHECK(h3d_target_alloc((void **)&d_cface_lo, (size_t)ghost * n * n * 2, o.device));
+      CHECK(h3d_target_alloc((void **)&d_cface_hi, (size_t)ghost * n * n * 2, o.device));
+    }
   }
 
   printf("\n=== SIMULATION STARTS ===\n");
@@ -480,6 +486,12 @@ int main(int argc, char **argv)
       double *t = uo; uo = un; un = t;
     }
     in_b = (uo == d_ub);
+  } else if (o.slab_parts) {
+    /* FP32 slab: passes of `ghost` steps, one band exchange per pass */
+    CHECK(h3d_run_slab_pass_f32(d_ua, d_ub, o.bf16 ? NULL : d_kap, o.bf16 ? d_kap : NULL,
+                                d_ca, d_cb, &gd, f32factor, mode, ghost, (float *)d_face_lo,
+                                (float *)d_face_hi, d_cface_lo, d_cface_hi, steps, &tune,
+                                o.graph, stream, &dev_ms, &in_b));
   } else {
     CHECK(h3d_run_f32(d_ua, d_ub, o.bf16 ? NULL : d_kap, o.bf16 ? d_kap : NULL,
                       d_ca, d_cb, &gd, f32factor, steps, mode, &tune, o.graph,
@@ -519,7 +531,11 @@ int main(int argc, char **argv)
     dev_upload(rk, kap, &gd, 8, o.device);
     int rb = 0;
     float rms = 0;
-    CHECK(h3d_run_f64(r0.p, r1.p, NULL, rk.p, &gd, factor, steps, NULL, 0, stream, &rms, &rb));
+    if (o.slab_parts) /* the same slab, stand-in exchange and pass length */
+      CHECK(h3d_run_slab_pass_f64(r0.p, r1.p, NULL, rk.p, &gd, factor, ghost, d_face_lo,
+                                  d_face_hi, steps, NULL, 0, stream, &rms, &rb));
+    else
+      CHECK(h3d_run_f64(r0.p, r1.p, NULL, rk.p, &gd, factor, steps, NULL, 0, stream, &rms, &rb));
     double m[9];
     CHECK(h3d_compare_f32(rb ? r1.p : r0.p, (const float *)d_res, &gd, 1e-6, 1e-5, m, stream));
     printf("vs FP64: rel_l2 = %.3e  max_abs = %.3e  max_rel = %.3e  rms = %.3e  %s\n",
@@ -556,6 +572,8 @@ int main(int argc, char **argv)
   if (fcb.base) h3d_target_free(fcb.base, o.device);
   if (d_face_lo) h3d_target_free(d_face_lo, o.device);
   if (d_face_hi) h3d_target_free(d_face_hi, o.device);
+  if (d_cface_lo) h3d_target_free(d_cface_lo, o.device);
+  if (d_cface_hi) h3d_target_free(d_cface_hi, o.device);
   h3d_stream_destroy(stream);
   free(u0);
   free(kap);

@@ -168,6 +168,32 @@ def test_cli_deep_slab_equals_oracle(gpu, oracle, tmp_path, mode, part):
     assert got.size == want.size and np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("prec,ghost,part", [("f32c", 1, 1), ("f32c", 3, 0), ("f32c16", 2, 3),
+                                             ("f32", 2, 2)])
+def test_cli_fp32_slab_equals_oracle(gpu, oracle, tmp_path, prec, ghost, part):
+    """An FP32 slab (--slab 4:p --precision ... --kblock K): passes of K
+    steps with bands of K FP32 planes (and FP16 residual planes in f32c16)
+    exchanged after each; the dump equals the oracle stepping the same
+    initial slab, rounded to FP32, with the same band copies, and --check
+    compares against the FP64 slab with the same exchange."""
+    from test_gpu_f32_slab import standin_oracle_f32
+    n = 34
+    args = [str(n), "--steps", "5", "--mode", "seeded", "--precision", prec,
+            "--slab", f"4:{part}", "--check"]
+    if ghost > 1:
+        args += ["--kblock", str(ghost)]
+    out = run(args, str(tmp_path))
+    assert re.search(r"^vs FP64: .* PASS$", out, re.M), out
+    steps = int(re.search(r"^(\d+) timesteps complete$", out, re.M).group(1))
+    g, u, _, k, factor = cli_deep_slab_initial(gpu, n, 4, part, ghost, True)
+    F = dict(g=g, u=u.astype(np.float32), k=k.astype(np.float32), bits=None,
+             factor=float(np.float32(factor)))
+    want, _ = standin_oracle_f32(oracle, F, g, ghost, steps, {"f32": 0, "f32c": 1,
+                                                             "f32c16": 2}[prec])
+    got = np.loadtxt(tmp_path / f"u_h3d_slab{part}of4.txt")
+    assert got.size == want.size and np.array_equal(got, want.astype(np.float64))
+
+
 def test_cli_autotune(tmp_path):
     out = run(["70", "--steps", "4", "--mode", "seeded", "--autotune"], str(tmp_path),
               write=False)
