@@ -105,6 +105,19 @@ def test_pass_matches_oracle_bitwise(gpu, oracle, case, mode, bf16):
         assert np.array_equal(got_c[inner], want_c[inner])
 
 
+@pytest.mark.parametrize("mode", [0, 1, 2])
+@pytest.mark.parametrize("kb", [1, 2, 3, 5])
+@pytest.mark.parametrize("shape", [(3, 3, 3), (3, 5, 4), (4, 3, 6)], ids=str)
+def test_pass_minimum_grids(gpu, oracle, shape, kb, mode):
+    """Grids of 3 and 4 points on an axis: single steps and passes of 2, 3
+    and 5 steps on each depth's default shape give the oracle's bits."""
+    P = problem(gpu, *shape)
+    steps = 2 * kb + 1
+    got_u, _ = gpu_run(gpu, P, steps, mode, gpu.Tune(0, 0, 0, kb))
+    want_u, _ = oracle_run(oracle, P, steps, kb, mode)
+    assert got_u.tobytes() == want_u.tobytes()
+
+
 def test_plain_fused_pass_equals_single_steps(gpu):
     P = problem(gpu, 100, 60, 40, seed=2)
     a, _ = gpu_run(gpu, P, 8, 0, gpu.Tune(2, 8, 0, 4))

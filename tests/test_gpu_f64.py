@@ -274,6 +274,30 @@ PASS_CASES = [
 ]
 
 
+@pytest.mark.parametrize("kb", [2, 3, 4])
+@pytest.mark.parametrize("shape,with_rhs", [((3, 3, 3), False), ((3, 5, 4), True),
+                                            ((4, 3, 6), False), ((6, 4, 3), True)], ids=str)
+def test_fused_pass_minimum_grids(gpu, oracle, shape, with_rhs, kb):
+    """Grids of 3 and 4 points on an axis (one or two interior points): a
+    tile and its warm-up planes reach far past the array on every side, and
+    passes on each depth's default shape still give the oracle's bits with
+    the ring untouched."""
+    nx, ny, nz = shape
+    g, u, k = seeded(gpu, nx, ny, nz, kind=1)
+    rhs = np.random.default_rng(5).uniform(-1e-4, 1e-4, g.elems) if with_rhs else None
+    factor = 0.9 * 2.0 / oracle.face_sum_max(k, nx, ny, nz) / 2.0
+    steps = 2 * kb + 1
+    want = oracle_steps(oracle, g, u, k, factor, steps, rhs=rhs)
+    da, db, dk = (gpu.DeviceArray.from_host(x) for x in (u, u, k))
+    dr = gpu.DeviceArray.from_host(rhs) if with_rhs else None
+    ms, fb = gpu.C.c_float(), gpu.C.c_int()
+    t = gpu.Tune(0, 0, 0, kb)
+    gpu.call("h3d_run_f64", da.ptr, db.ptr, dr.ptr if dr else None, dk.ptr, gpu.C.byref(g),
+             factor, steps, gpu.C.byref(t), 0, None, gpu.C.byref(ms), gpu.C.byref(fb))
+    got = (db if fb.value else da).download()
+    assert np.array_equal(got, want)
+
+
 @pytest.mark.parametrize("extra", [0, 1], ids=["2k+1", "3k+k-1"])
 @pytest.mark.parametrize("case", PASS_CASES, ids=str)
 def test_fused_pass_matches_oracle_bitwise(gpu, oracle, case, extra):
