@@ -379,9 +379,9 @@ def leg_cfg3(bw, stream):
     lo, hi = h3d.DeviceArray(n * n * 8), h3d.DeviceArray(n * n * 8)
     ms, fb = h3d.C.c_float(), h3d.C.c_int()
     args = (da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor, lo.ptr, hi.ptr)
-    t = h3d.Tune()
-    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, 1, stream.ptr,
-             h3d.C.byref(t))  # the slab step runs single steps
+    t = h3d.Tune()  # the slab step runs single steps: the slab at ghost depth 1
+    h3d.call("h3d_autotune_slab", 0, da.ptr, dk.ptr, 0, None, h3d.C.byref(g), 0, 1, lo.ptr,
+             hi.ptr, None, None, stream.ptr, h3d.C.byref(t))
     h3d.call("h3d_run_slab_f64", *args, 10, h3d.C.byref(t), 1, stream.ptr, h3d.C.byref(ms),
              h3d.C.byref(fb))
     h3d.call("h3d_run_slab_f64", *args, steps, h3d.C.byref(t), 1, stream.ptr,
@@ -414,7 +414,7 @@ def slab_deep(n, owned, gk_owned, K, factor, steps, bw, stream):
     2K) points): fused passes of K steps, bands of K planes exchanged in
     HBM once per pass, the interior overlapped with the bands.  Same update
     count and factor as the one-ghost slab; the interior pass shape comes
-    from the autotuner at pass depth K."""
+    from the slab autotuner (timed on the slab pass itself) at depth K."""
     planes = owned + 2 * K
     g = h3d.pitched_grid(n, n, planes)
     u, k = h3d.gen_seeded(g, gk_owned - K, n, n, n, 1, 20260101)
@@ -427,8 +427,8 @@ def slab_deep(n, owned, gk_owned, K, factor, steps, bw, stream):
     lo, hi = h3d.DeviceArray(K * n * n * 8), h3d.DeviceArray(K * n * n * 8)
     ms, fb = h3d.C.c_float(), h3d.C.c_int()
     t = h3d.Tune()
-    h3d.call("h3d_autotune", 0, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 0, K, stream.ptr,
-             h3d.C.byref(t))
+    h3d.call("h3d_autotune_slab", 0, da.ptr, dk.ptr, 0, None, h3d.C.byref(g), 0, K, lo.ptr,
+             hi.ptr, None, None, stream.ptr, h3d.C.byref(t))
     args = (da.ptr, db.ptr, None, dk.ptr, h3d.C.byref(g), factor, K, lo.ptr, hi.ptr)
     h3d.call("h3d_run_slab_pass_f64", *args, 4 * K, h3d.C.byref(t), 1, stream.ptr,
              h3d.C.byref(ms), h3d.C.byref(fb))
@@ -460,8 +460,8 @@ def slab_deep_f32(n, owned, gk_owned, K, factor, steps, bw, stream):
     """The slab with FP32 fields (u and kappa rounded from the same seeded
     values), K ghost planes per side, compensated passes of K steps with the
     residual in registers (H3D_F32_CARRY), bands of K FP32 planes exchanged
-    in HBM once per pass; the interior pass shape from the autotuner at
-    depth K."""
+    in HBM once per pass; the interior pass shape from the slab autotuner
+    at depth K."""
     planes = owned + 2 * K
     g = h3d.pitched_grid(n, n, planes)
     u, k = h3d.gen_seeded(g, gk_owned - K, n, n, n, 1, 20260101)
@@ -475,8 +475,8 @@ def slab_deep_f32(n, owned, gk_owned, K, factor, steps, bw, stream):
     lo, hi = h3d.DeviceArray(K * n * n * 4), h3d.DeviceArray(K * n * n * 4)
     ms, fb = h3d.C.c_float(), h3d.C.c_int()
     t = h3d.Tune()
-    h3d.call("h3d_autotune", 1, da.ptr, db.ptr, dk.ptr, 0, h3d.C.byref(g), 1, K, stream.ptr,
-             h3d.C.byref(t))
+    h3d.call("h3d_autotune_slab", 1, da.ptr, dk.ptr, 0, None, h3d.C.byref(g), 1, K, lo.ptr,
+             hi.ptr, None, None, stream.ptr, h3d.C.byref(t))
     f32f = h3d.C.c_float(np.float32(factor))
     args = (da.ptr, db.ptr, dk.ptr, None, None, None, h3d.C.byref(g), f32f, 1, K, lo.ptr,
             hi.ptr, None, None)

@@ -244,6 +244,25 @@ int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_f32,
                           uint16_t *cface_hi, int steps, const h3d_tune *tune,
                           int graph, void *stream, float *ms, int *final_in_b);
 
+/* The `tune` of a slab's interior launch for the slab functions above,
+ * timed on the slab pass itself (no reference counterpart).  kind 0: FP64
+ * slab (kappa FP64; mode 0 without rhs, 1 with rhs; ghost 1..4, 1 for
+ * h3d_run_slab_f64); kind 1: FP32 slab (kappa FP32, or BF16 bits when
+ * kappa_bf16; mode as h3d_pass_f32; ghost 1..8; comp_a, the FP16 residual
+ * field, and cface_lo / cface_hi in mode H3D_F32_COMP16).  The rows and
+ * waves come from h3d_autotune of the interior launch alone on a grid of
+ * exactly its planes; the z chunk from timing whole passes of `ghost`
+ * steps (bands, exchange and the interior on a second stream) for that
+ * shape's candidate chunks: alone, the interior may prefer another chunk
+ * than beside the bands.  Reads u_a, kappa and comp_a only (passes run on
+ * scratch copies); face_lo / face_hi (and cface_*) are used as workspace.
+ * stream must be a created stream.  Results are cached per (kind, grid,
+ * mode, kappa format, ghost); any tune gives the same bits. */
+int h3d_autotune_slab(int kind, const void *u_a, const void *kappa, int kappa_bf16,
+                      const void *comp_a, const h3d_grid *g, int mode, int ghost,
+                      void *face_lo, void *face_hi, void *cface_lo, void *cface_hi,
+                      void *stream, h3d_tune *out);
+
 /* FP64 diagnostics over the whole array (ring included), deterministic:
  * out[0] = sum u (checksum), out[1] = sum u^2, out[2] = max |u|,
  * out[3] = number of non-finite values.  Host `out`; synchronises stream. */

@@ -22,6 +22,10 @@
 // bands of gh planes once per pass -- the same bytes per step as the
 // one-plane exchange, in 1/gh of the messages.  With gh = 1 this is the
 // reference's step above.
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
 #include "h3d_internal.h"
 
 namespace {
@@ -306,4 +310,178 @@ extern "C" int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_
                                          factor, mode, ghost, ns, face_lo, face_hi, cface_lo,
                                          cface_hi, tune, s, s2, fork, join);
                   });
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning a slab's interior launch on the slab pass itself.
+//
+// The interior launch shares the GPU with the band passes and the exchange
+// (they run on the caller's stream while it runs on the second one), and
+// marches only the planes between the bands.  Tuned alone on the whole
+// slab, the FP32 3-step interior of the 1026^2 x 262 slab took 3 z chunks
+// (87 planes) and ran 813-817 GLUP/s; two chunks of 125 planes ran 872
+// inside the slab pass.  So the shape (rows, waves) comes from the pass
+// tuned alone on a grid of exactly the interior planes, and its z chunk
+// from timing whole slab passes for that shape's candidate chunks.
+
+namespace {
+
+struct SlabKey {
+  int kind, nx, ny, nz, sy, mode, bf16, ghost;
+  long long sz;
+  bool operator==(const SlabKey &o) const {
+    return kind == o.kind && nx == o.nx && ny == o.ny && nz == o.nz && sy == o.sy &&
+           mode == o.mode && bf16 == o.bf16 && ghost == o.ghost && sz == o.sz;
+  }
+};
+std::mutex slab_tune_mu;
+std::vector<std::pair<SlabKey, h3d_tune>> slab_tune_cache;
+
+// A device copy of `bytes` bytes of src at src's offset within 256 bytes
+// (the tiles' line crossings follow the address); *base is what to free.
+int scratch_copy(const void *src, size_t bytes, void **base, void **p, hipStream_t s) {
+  *base = *p = nullptr;
+  H3D_TRY(hipMalloc(base, bytes + 256));
+  *p = (void *)((uintptr_t)*base + (uintptr_t)src % 256);  // allocations are 256-B aligned
+  H3D_TRY(hipMemcpyAsync(*p, src, bytes, hipMemcpyDeviceToDevice, s));
+  return H3D_OK;
+}
+
+}  // namespace
+
+extern "C" int h3d_autotune_slab(int kind, const void *u_a, const void *kappa, int kappa_bf16,
+                                 const void *comp_a, const h3d_grid *g, int mode, int ghost,
+                                 void *face_lo, void *face_hi, void *cface_lo,
+                                 void *cface_hi, void *stream, h3d_tune *out) {
+  if (!u_a || !kappa || !out || !h3d_grid_ok(g)) return H3D_EARG;
+  if (kind == 0 ? (mode < 0 || mode > 1 || kappa_bf16) : (kind != 1 || mode < 0 || mode > 2))
+    return H3D_EARG;
+  if (ghost < 1 || ghost > (kind ? 8 : 4) || g->nz < 3 * ghost || g->nz < 4) return H3D_EARG;
+  if (!face_lo || !face_hi || face_lo == face_hi) return H3D_EARG;
+  const bool c16 = kind == 1 && mode == H3D_F32_COMP16;
+  if (c16 && (!comp_a || !cface_lo || !cface_hi || cface_lo == cface_hi)) return H3D_EARG;
+  hipStream_t s = h3d_stream(stream);
+  if (!s) return H3D_EARG;  // the slab pass forks from a created stream
+  const SlabKey key{kind, g->nx, g->ny, g->nz, g->sy, mode, kappa_bf16 ? 1 : 0, ghost, g->sz};
+  {
+    std::lock_guard<std::mutex> lk(slab_tune_mu);
+    for (auto &e : slab_tune_cache)
+      if (e.first == key) { *out = e.second; return H3D_OK; }
+  }
+  // 1. the shape: the interior launch tuned alone on a grid of exactly its
+  //    planes (2*ghost .. nz-2*ghost-1, plus one on each side), at the same
+  //    addresses; the whole slab when there is no interior launch
+  const size_t esz = kind ? sizeof(float) : sizeof(double);
+  const size_t ksz = kind ? (kappa_bf16 ? sizeof(uint16_t) : sizeof(float)) : sizeof(double);
+  const int nzv = g->nz - 4 * ghost + 2;
+  h3d_tune base{};
+  std::vector<int> zcs;
+  if (nzv >= 3) {
+    const long long first = (long long)(2 * ghost - 1) * g->sz;
+    h3d_grid gv = *g;
+    gv.nz = nzv;
+    const int r0 = h3d_autotune(kind, (void *)((const char *)u_a + first * esz), nullptr,
+                                (const char *)kappa + first * ksz, kappa_bf16, &gv, mode, ghost,
+                                stream, &base);
+    if (r0 != H3D_OK) return r0;
+    zcs.push_back(base.zchunk);
+    // 2. the other z chunks for that shape over the interior planes: the
+    //    round model's best three and one chunk of the whole interior
+    const int planes = nzv - 2;
+    std::vector<int> more;
+    if (ghost == 1) {
+      more = {planes, 256, 128, 64};
+    } else {
+      int occ = 0, z[3], got = 0;
+      long long tiles = 0;
+      const int r = kind ? h3d_pass_f32_geometry(&gv, ghost, base.rows, base.waves, mode,
+                                                 kappa_bf16, &occ, &tiles)
+                         : h3d_pass_f64_geometry(&gv, ghost, base.rows, base.waves, mode, &occ,
+                                                 &tiles);
+      if (r == H3D_OK && h3d_round_zchunks(planes, ghost, tiles, occ, z, 3, &got) == H3D_OK)
+        more.assign(z, z + got);
+      if (planes <= 256) more.push_back(planes);
+    }
+    for (int z : more)
+      if (z >= 1 && z <= planes && std::find(zcs.begin(), zcs.end(), z) == zcs.end())
+        zcs.push_back(z);
+  } else {
+    const int r0 = h3d_autotune(kind, (void *)u_a, nullptr, kappa, kappa_bf16, g, mode, ghost,
+                                stream, &base);
+    if (r0 != H3D_OK) return r0;
+  }
+  h3d_tune best = base;
+  int rc = H3D_OK;
+  if (zcs.size() > 1) {
+    // 3. whole slab passes (bands, exchange, the interior on a second
+    //    stream) through scratch copies of u_a (and comp_a): the caller's
+    //    fields are only read; the face buffers are workspace
+    const size_t cells = h3d_grid_elems(g);
+    void *ub[2] = {nullptr, nullptr}, *u[2] = {nullptr, nullptr};
+    void *cb[2] = {nullptr, nullptr}, *c[2] = {nullptr, nullptr};
+    hipStream_t s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, e0 = nullptr, e1 = nullptr;
+    for (int i = 0; i < 2 && rc == H3D_OK; ++i) {
+      rc = scratch_copy(u_a, cells * esz, &ub[i], &u[i], s);
+      if (rc == H3D_OK && c16) rc = scratch_copy(comp_a, cells * 2, &cb[i], &c[i], s);
+    }
+    hipError_t he = hipSuccess;
+    if (rc == H3D_OK) he = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&fork, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&join, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) rc = (int)he;
+    const double *rhs = kind == 0 && mode == 1 ? (const double *)u_a : nullptr;  // stand-in
+    const float *kf = kappa_bf16 ? nullptr : (const float *)kappa;
+    const uint16_t *kb16 = kappa_bf16 ? (const uint16_t *)kappa : nullptr;
+    auto pass = [&](const h3d_tune &t, int j) {  // pass j: buffer j%2 -> (j+1)%2
+      const int i = j % 2, o = 1 - i;
+      if (kind == 0)
+        return slab_pass_f64((double *)u[o], (const double *)u[i], rhs, (const double *)kappa,
+                             g, 1e-3, ghost, ghost, (double *)face_lo, (double *)face_hi, &t,
+                             s, s2, fork, join);
+      return slab_pass_f32((float *)u[o], (const float *)u[i], kf, kb16, (uint16_t *)c[o],
+                           (const uint16_t *)c[i], g, 1e-3f, mode, ghost, ghost,
+                           (float *)face_lo, (float *)face_hi, (uint16_t *)cface_lo,
+                           (uint16_t *)cface_hi, &t, s, s2, fork, join);
+    };
+    float best_ms = 1e30f;
+    for (size_t k = 0; rc == H3D_OK && k < zcs.size(); ++k) {
+      h3d_tune t = base;
+      t.zchunk = zcs[k];
+      rc = pass(t, 0);  // warm-up
+      // best of three samples of four passes
+      for (int r = 0; rc == H3D_OK && r < 3; ++r) {
+        he = hipEventRecord(e0, s);
+        for (int j = 0; he == hipSuccess && rc == H3D_OK && j < 4; ++j) rc = pass(t, j);
+        if (rc != H3D_OK) break;
+        if (he == hipSuccess) he = hipEventRecord(e1, s);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (he == hipSuccess) he = hipEventElapsedTime(&ms, e0, e1);
+        if (he != hipSuccess) { rc = (int)he; break; }
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = t;
+        }
+      }
+    }
+    (void)hipStreamSynchronize(s);
+    if (s2) {
+      (void)hipStreamSynchronize(s2);
+      (void)hipStreamDestroy(s2);
+    }
+    for (hipEvent_t e : {fork, join, e0, e1})
+      if (e) (void)hipEventDestroy(e);
+    for (void *p : {ub[0], ub[1], cb[0], cb[1]})
+      if (p) (void)hipFree(p);
+  }
+  if (rc != H3D_OK) return rc;
+  {
+    std::lock_guard<std::mutex> lk(slab_tune_mu);
+    slab_tune_cache.push_back({key, best});
+  }
+  *out = best;
+  return H3D_OK;
 }

@@ -98,6 +98,7 @@ SIGNATURES = {
     "h3d_run_f32": (_I, [_P, _P, _P, _P, _P, _P, _GP, C.c_float, _I, _I, _TP,
                          _I, _P, C.POINTER(C.c_float), C.POINTER(_I)]),
     "h3d_autotune": (_I, [_I, _P, _P, _P, _I, _GP, _I, _I, _P, _TP]),
+    "h3d_autotune_slab": (_I, [_I, _P, _P, _I, _P, _GP, _I, _I, _P, _P, _P, _P, _P, _TP]),
     "h3d_device_count": (_I, [C.POINTER(_I)]),
     "h3d_target_alloc": (_I, [C.POINTER(_P), C.c_size_t, _I]),
     "h3d_target_free": (_I, [_P, _I]),

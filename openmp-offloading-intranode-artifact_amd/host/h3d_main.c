@@ -433,17 +433,6 @@ int main(int argc, char **argv)
     const double interior = (double)(gd.nx - 2) * (gd.ny - 2) * (gd.nz - 2);
     o.autotune = interior >= 128.0 * 128.0 * 128.0 && o.tune.rows == 0 && o.tune.waves == 0;
   }
-  if (o.autotune) {
-    h3d_tune t;
-    /* FP64: mode 1 tunes shapes that hold rhs; a slab runs passes of as
-     * many steps as it has ghost planes per side */
-    CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &gd,
-                       f32 ? mode : (rhs ? 1 : 0),
-                       o.slab_parts ? ghost : o.tune.kblock, stream, &t));
-    tune = t;
-    printf("Autotuned: rows=%d waves=%d zchunk=%d kblock=%d\n", tune.rows,
-           tune.waves, tune.zchunk, tune.kblock);
-  }
 
   /* slab: face (band) buffers for the in-HBM exchange, sized for FP64;
    * the FP16 residual's bands in f32c16 */
@@ -457,6 +446,23 @@ int main(int argc, char **argv)
       CHECK(h3d_target_alloc((void **)&d_cface_hi, (size_t)ghost * n * n * 2, o.device));
     }
   }
+  if (o.autotune) {
+    h3d_tune t;
+    /* FP64: mode 1 tunes shapes that hold rhs.  A slab runs passes of as
+     * many steps as it has ghost planes per side, its interior beside the
+     * bands: tuned on the slab pass itself */
+    if (o.slab_parts)
+      CHECK(h3d_autotune_slab(f32 ? 1 : 0, d_ua, d_kap, o.bf16, d_ca, &gd,
+                              f32 ? mode : (rhs ? 1 : 0), ghost, d_face_lo, d_face_hi,
+                              d_cface_lo, d_cface_hi, stream, &t));
+    else
+      CHECK(h3d_autotune(f32 ? 1 : 0, d_ua, d_ub, d_kap, o.bf16, &gd,
+                         f32 ? mode : (rhs ? 1 : 0), o.tune.kblock, stream, &t));
+    tune = t;
+    printf("Autotuned: rows=%d waves=%d zchunk=%d kblock=%d\n", tune.rows,
+           tune.waves, tune.zchunk, tune.kblock);
+  }
+
 
   printf("\n=== SIMULATION STARTS ===\n");
   CHECK(h3d_device_synchronize(o.device));

@@ -194,6 +194,19 @@ def test_cli_fp32_slab_equals_oracle(gpu, oracle, tmp_path, prec, ghost, part):
     assert got.size == want.size and np.array_equal(got, want.astype(np.float64))
 
 
+@pytest.mark.parametrize("prec", ["f64", "f32c16"])
+def test_cli_slab_autotune_keeps_bits(tmp_path, prec):
+    """--autotune on a slab tunes the interior on the slab pass itself
+    (h3d_autotune_slab) and changes no bit of the result."""
+    args = ["34", "--steps", "5", "--mode", "seeded", "--precision", prec, "--slab", "4:1",
+            "--kblock", "2"]
+    tuned = run(args + ["--autotune"], str(tmp_path), write=False)
+    assert re.search(r"^Autotuned: .*kblock=2$", tuned, re.M), tuned
+    plain = run(args + ["--no-autotune"], str(tmp_path), write=False)
+    line = lambda out: next(ln for ln in out.splitlines() if ln.startswith("checksum ="))
+    assert line(tuned) == line(plain)
+
+
 def test_cli_autotune(tmp_path):
     out = run(["70", "--steps", "4", "--mode", "seeded", "--autotune"], str(tmp_path),
               write=False)

@@ -267,3 +267,85 @@ def test_f32_slab_arguments(gpu):
     # both kappa forms at once
     assert L.h3d_slab_pass_f32(a.ptr, b.ptr, k.ptr, k.ptr, None, None, gpu.C.byref(g), f, 1,
                                2, 2, lo.ptr, hi.ptr, None, None, None, None) == EARG
+
+
+SLAB_TUNE_CASES = [
+    # kind (0 FP64, 1 FP32), mode, ghost, bf16 kappa
+    (1, 2, 3, False), (1, 1, 2, True), (1, 0, 1, False), (0, 0, 2, False), (0, 1, 1, False),
+    (0, 0, 3, False),
+]
+
+
+@pytest.mark.parametrize("kind,mode,ghost,bf16", SLAB_TUNE_CASES, ids=str)
+def test_autotune_slab_reads_only_and_keeps_bits(gpu, kind, mode, ghost, bf16):
+    """h3d_autotune_slab times whole slab passes on scratch copies: the
+    caller's u, kappa and residual field are unchanged, the tune has the
+    slab's depth, and a run with it equals a run on the default shape bit
+    for bit."""
+    n, nz, steps = 70, 40, 2 * ghost + 1
+    g = gpu.Grid(n, n, nz, n + 3, (n + 3) * n)
+    u, k = gpu.gen_seeded(g, 0, n, n, nz, 1, 21)
+    u = np.where(u == 0.0, 0.5, u)  # ghosts hold data
+    esz = 8 if kind == 0 else 4
+    uh = u.astype(np.float64 if kind == 0 else np.float32)
+    if kind == 1 and bf16:
+        kh, _ = gpu.quantize_bf16(k)
+    else:
+        kh = k.astype(uh.dtype)
+    ch = np.random.default_rng(4).integers(0, 1 << 10, g.elems).astype(np.uint16)
+    rhs = np.random.default_rng(6).uniform(-1e-4, 1e-4, g.elems) if kind == 0 and mode else None
+    da, dk, dc = (gpu.DeviceArray.from_host(x) for x in (uh, kh, ch))
+    lo, hi = gpu.DeviceArray(ghost * n * n * esz), gpu.DeviceArray(ghost * n * n * esz)
+    clo, chi = gpu.DeviceArray(ghost * n * n * 2), gpu.DeviceArray(ghost * n * n * 2)
+    s = gpu.Stream()
+    t = gpu.Tune()
+    c16 = kind == 1 and mode == 2
+    gpu.call("h3d_autotune_slab", kind, da.ptr, dk.ptr, int(bf16), dc.ptr if c16 else None,
+             gpu.C.byref(g), mode, ghost, lo.ptr, hi.ptr, clo.ptr if c16 else None,
+             chi.ptr if c16 else None, s.ptr, gpu.C.byref(t))
+    assert da.download().tobytes() == uh.tobytes()
+    assert dk.download().tobytes() == kh.tobytes() and dc.download().tobytes() == ch.tobytes()
+    assert t.kblock == ghost and t.rows > 0 and t.waves > 0 and t.zchunk > 0
+
+    def run(tune):
+        a, b = gpu.DeviceArray.from_host(uh), gpu.DeviceArray.from_host(uh)
+        ca, cb = gpu.DeviceArray.from_host(ch), gpu.DeviceArray.from_host(ch)
+        ms, fb = gpu.C.c_float(), gpu.C.c_int()
+        tp = gpu.C.byref(tune) if tune is not None else None
+        if kind == 1:
+            kf, kb = (None, dk.ptr) if bf16 else (dk.ptr, None)
+            gpu.call("h3d_run_slab_pass_f32", a.ptr, b.ptr, kf, kb, ca.ptr, cb.ptr,
+                     gpu.C.byref(g), gpu.C.c_float(1e-3), mode, ghost, lo.ptr, hi.ptr,
+                     clo.ptr if c16 else None, chi.ptr if c16 else None, steps, tp, 0, s.ptr,
+                     gpu.C.byref(ms), gpu.C.byref(fb))
+        else:
+            dr = gpu.DeviceArray.from_host(rhs) if rhs is not None else None
+            gpu.call("h3d_run_slab_pass_f64", a.ptr, b.ptr, dr.ptr if dr else None, dk.ptr,
+                     gpu.C.byref(g), 1e-3, ghost, lo.ptr, hi.ptr, steps, tp, 0, s.ptr,
+                     gpu.C.byref(ms), gpu.C.byref(fb))
+        out = (b if fb.value else a).download().tobytes()
+        return out + ((cb if fb.value else ca).download().tobytes() if c16 else b"")
+
+    assert run(t) == run(None)
+
+
+def test_autotune_slab_arguments(gpu):
+    n, nz = 20, 12
+    g = gpu.Grid.dense(n, n, nz)
+    a = gpu.DeviceArray(g.elems * 8)
+    k = gpu.DeviceArray(g.elems * 8)
+    lo, hi = gpu.DeviceArray(4 * n * n * 8), gpu.DeviceArray(4 * n * n * 8)
+    L = gpu.lib()
+    EARG = gpu.H3D_EARG
+    s = gpu.Stream()
+    t = gpu.Tune()
+    args = lambda kind, mode, ghost, stream=s.ptr, comp=None: (
+        kind, a.ptr, k.ptr, 0, comp, gpu.C.byref(g), mode, ghost, lo.ptr, hi.ptr, None, None,
+        stream, gpu.C.byref(t))
+    assert L.h3d_autotune_slab(*args(0, 0, 2, stream=None)) == EARG  # needs a created stream
+    assert L.h3d_autotune_slab(*args(0, 0, 0)) == EARG               # ghost depth 0
+    assert L.h3d_autotune_slab(*args(0, 0, 5)) == EARG               # FP64 ghosts 1..4
+    assert L.h3d_autotune_slab(*args(0, 2, 2)) == EARG               # FP64 modes 0, 1
+    assert L.h3d_autotune_slab(*args(1, 2, 2)) == EARG               # COMP16 needs residuals
+    assert L.h3d_autotune_slab(*args(1, 1, 5)) == EARG               # slab too thin
+    assert L.h3d_autotune_slab(*args(2, 0, 2)) == EARG               # no such kind
