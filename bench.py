@@ -101,14 +101,23 @@ def profile_entry(key, alg_bytes=None):
     the same kernel instantiation and workgroup size with the nearest grid,
     else the same kernel family and depth (template arguments but the tile
     rows and waves) with the nearest grid, marked "same_shape": False.  Its
-    key is returned as "profiled_launch".  None when neither exists."""
+    key is returned as "profiled_launch".  None when neither exists.
+
+    Two problems can launch the same kernel on the same grid (the 512^3
+    field and the 1026^2 x 262 slab's interior both take 720 workgroups of
+    the 3-step pass): the table then holds the second under
+    "<key>@<algorithmic bytes>", and an entry counts only for the launch
+    whose algorithmic bytes it was measured with."""
     path = os.path.join(REPO, "profiles", "traffic.json")
     try:
         table = json.load(open(path))
     except (OSError, ValueError):
         return None
-    if key in table:
-        return dict(table[key], profiled_launch=key, same_shape=True)
+    same = (lambda e: alg_bytes is None
+            or e.get("algorithmic_bytes_per_launch") == int(alg_bytes))
+    for k in ((f"{key}@{int(alg_bytes)}",) if alg_bytes is not None else ()) + (key,):
+        if k in table and same(table[k]):
+            return dict(table[k], profiled_launch=k, same_shape=True)
     if alg_bytes is None or "|" not in key:
         return None
     kern, grid, wg = key.split("|")
@@ -120,11 +129,13 @@ def profile_entry(key, alg_bytes=None):
         a = [x.strip() for x in args.rstrip(">").split(",")]
         return (name, a[0], tuple(a[3:])) if "_pass_" in name and len(a) >= 3 else (name, tuple(a))
 
-    same_problem = [(k, e) for k, e in table.items() if k.count("|") == 2
-                    and e.get("algorithmic_bytes_per_launch") == int(alg_bytes)]
-    for exact, match in ((True, lambda k: k.split("|")[0] == kern and k.split("|")[2] == wg),
-                         (False, lambda k: family(k.split("|")[0]) == family(kern))):
-        near = [(abs(int(k.split("|")[1]) - int(grid)), k) for k, _ in same_problem if match(k)]
+    def parts(k):  # kernel, grid, workgroup of a table key ("@bytes" dropped)
+        return k.split("@")[0].split("|")
+
+    same_problem = [(k, e) for k, e in table.items() if k.count("|") == 2 and same(e)]
+    for exact, match in ((True, lambda k: parts(k)[0] == kern and parts(k)[2] == wg),
+                         (False, lambda k: family(parts(k)[0]) == family(kern))):
+        near = [(abs(int(parts(k)[1]) - int(grid)), k) for k, _ in same_problem if match(k)]
         if near:
             k = min(near)[1]
             return dict(table[k], profiled_launch=k, same_shape=exact)
