@@ -517,6 +517,15 @@ double h3d_face_sum_max_host(const double *kappa, const h3d_grid *g);
  * reference input).  n values. */
 int h3d_quantize_bf16(const double *kappa, uint16_t *bits, size_t n,
                       double *scale);
+/* The two halves of h3d_quantize_bf16, for fields split across slabs:
+ * every slab must store its kappa with the whole field's scale, or its bits
+ * differ from the whole field's.  h3d_bf16_scale gives the power-of-two
+ * scale of n values (2^e for max|kappa| = m * 2^e, m in [0.5, 1); 1 for all
+ * zeros); h3d_quantize_bf16_scaled writes bf16(kappa/scale) with a given
+ * power-of-two scale (H3D_EARG for another scale, non-finite values or
+ * values BF16 cannot hold). */
+int h3d_bf16_scale(const double *kappa, size_t n, double *scale);
+int h3d_quantize_bf16_scaled(const double *kappa, uint16_t *bits, size_t n, double scale);
 
 /* `%.17g` one value per line, k,j,i order, all nz*ny*nx points
  * (1-openmp.c:41-51). */

@@ -245,10 +245,9 @@ static uint16_t f32_to_bf16_rne(float x)
   return (uint16_t)(w >> 16);
 }
 
-int h3d_quantize_bf16(const double *kappa, uint16_t *bits, size_t n,
-                      double *scale)
+int h3d_bf16_scale(const double *kappa, size_t n, double *scale)
 {
-  if (!kappa || !bits || !scale || n == 0) return H3D_EARG;
+  if (!kappa || !scale || n == 0) return H3D_EARG;
   double mx = 0.0;
   int bad = 0;
 #pragma omp parallel for schedule(static) reduction(max : mx) reduction(| : bad)
@@ -259,11 +258,35 @@ int h3d_quantize_bf16(const double *kappa, uint16_t *bits, size_t n,
   if (bad) return H3D_EARG;
   int e = 0;
   if (mx > 0.0) frexp(mx, &e); /* mx = m * 2^e, m in [0.5, 1) */
-  const double s = ldexp(1.0, e);
-#pragma omp parallel for schedule(static)
-  for (size_t c = 0; c < n; c++) bits[c] = f32_to_bf16_rne((float)(kappa[c] / s));
-  *scale = s;
+  *scale = ldexp(1.0, e);
   return H3D_OK;
+}
+
+int h3d_quantize_bf16_scaled(const double *kappa, uint16_t *bits, size_t n, double scale)
+{
+  if (!kappa || !bits || n == 0 || !(scale > 0.0) || !isfinite(scale)) return H3D_EARG;
+  int e = 0;
+  if (frexp(scale, &e) != 0.5) return H3D_EARG; /* a power of two: division is exact */
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (size_t c = 0; c < n; c++) {
+    const double q = kappa[c] / scale;
+    if (!isfinite(q) || fabs(q) >= 0x1.fep127) bad = 1; /* bf16 holds it finitely */
+    else bits[c] = f32_to_bf16_rne((float)q);
+  }
+  return bad ? H3D_EARG : H3D_OK;
+}
+
+int h3d_quantize_bf16(const double *kappa, uint16_t *bits, size_t n,
+                      double *scale)
+{
+  if (!kappa || !bits || !scale || n == 0) return H3D_EARG;
+  double s = 0.0;
+  const int rc = h3d_bf16_scale(kappa, n, &s);
+  if (rc != H3D_OK) return rc;
+  const int rq = h3d_quantize_bf16_scaled(kappa, bits, n, s);
+  if (rq == H3D_OK) *scale = s;
+  return rq;
 }
 
 int h3d_write_dump(const char *fname, const double *u, const h3d_grid *g)

@@ -122,6 +122,8 @@ SIGNATURES = {
     "h3d_rng_uniform": (_D, [C.c_uint64, _I, _I, C.c_uint64]),
     "h3d_face_sum_max_host": (_D, [_P, _GP]),
     "h3d_quantize_bf16": (_I, [_P, _P, C.c_size_t, C.POINTER(_D)]),
+    "h3d_bf16_scale": (_I, [_P, C.c_size_t, C.POINTER(_D)]),
+    "h3d_quantize_bf16_scaled": (_I, [_P, _P, C.c_size_t, _D]),
     "h3d_write_dump": (_I, [C.c_char_p, _P, _GP]),
     "h3d_ensemble_create": (_I, [C.POINTER(_P), C.POINTER(Member), _I, _TP, _I]),
     "h3d_ensemble_run": (_I, [_P, _I, _I, _P, C.POINTER(C.c_float)]),
@@ -313,6 +315,16 @@ def quantize_bf16(kappa):
     s = C.c_double()
     call("h3d_quantize_bf16", ptr(kappa), ptr(bits), C.c_size_t(kappa.size), C.byref(s))
     return bits, s.value
+
+
+def quantize_bf16_scaled(kappa, scale):
+    """bf16(kappa/scale) with a given power-of-two scale (a slab of a field
+    takes the whole field's scale)."""
+    kappa = np.ascontiguousarray(kappa, dtype=np.float64)
+    bits = np.empty(kappa.size, dtype=np.uint16)
+    call("h3d_quantize_bf16_scaled", ptr(kappa), ptr(bits), C.c_size_t(kappa.size),
+         C.c_double(scale))
+    return bits
 
 
 def bf16_to_f64(bits, scale):

@@ -288,6 +288,7 @@ int main(int argc, char **argv)
   double *kap = (double *)xmalloc(cells * sizeof(double));
   double *rhs = NULL;
   double h, dt, factor, kappa_max;
+  double kscale_field = 0.0; /* BF16 kappa scale of the whole field (slabs) */
   if (!o.seeded) {
     CHECK(h3d_params_reference(n, &h, &dt, &factor));
     kappa_max = 0.95;
@@ -319,6 +320,9 @@ int main(int argc, char **argv)
       const h3d_grid gg = {n, n, n, n, (long long)n * n};
       CHECK(h3d_gen_seeded(NULL, kg, &gg, 0, n, n, n, o.kappa_kind, o.seed, 0));
       fmax = h3d_face_sum_max_host(kg, &gg);
+      /* and so must BF16 kappa's scale, or slabs' bits differ from the
+       * whole field's */
+      if (o.bf16) CHECK(h3d_bf16_scale(kg, total_points, &kscale_field));
       free(kg);
     } else {
       fmax = h3d_face_sum_max_host(kap, &g);
@@ -387,7 +391,12 @@ int main(int argc, char **argv)
     dev_upload(fb, h_u32, &gd, 4, o.device);
     if (o.bf16) {
       uint16_t *kb = (uint16_t *)xmalloc(cells * 2);
-      CHECK(h3d_quantize_bf16(kap, kb, cells, &kscale));
+      if (kscale_field > 0.0) {
+        kscale = kscale_field;
+        CHECK(h3d_quantize_bf16_scaled(kap, kb, cells, kscale));
+      } else {
+        CHECK(h3d_quantize_bf16(kap, kb, cells, &kscale));
+      }
       /* the FP64 check runs on exactly the kappa the FP32 run sees */
       for (size_t c = 0; c < cells; c++) {
         uint32_t w = (uint32_t)kb[c] << 16;

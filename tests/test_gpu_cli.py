@@ -194,6 +194,26 @@ def test_cli_fp32_slab_equals_oracle(gpu, oracle, tmp_path, prec, ghost, part):
     assert got.size == want.size and np.array_equal(got, want.astype(np.float64))
 
 
+@pytest.mark.parametrize("prec,part", [("f32c", 0), ("f32c16", 2)])
+def test_cli_fp32_slab_bf16_kappa_takes_the_field_scale(gpu, oracle, tmp_path, prec, part):
+    """--kappa-bf16 on a slab: kappa is stored with the whole field's BF16
+    scale (every slab of the split the same one, as the whole field run
+    has), and the dump equals the oracle's stand-in run on those bits."""
+    from test_gpu_f32_slab import standin_oracle_f32
+    n, ghost = 34, 2
+    out = run([str(n), "--steps", "5", "--mode", "seeded", "--precision", prec, "--kappa-bf16",
+               "--slab", f"4:{part}", "--kblock", str(ghost)], str(tmp_path))
+    steps = int(re.search(r"^(\d+) timesteps complete$", out, re.M).group(1))
+    g, u, _, k, factor = cli_deep_slab_initial(gpu, n, 4, part, ghost, True)
+    _, kg = gpu.gen_seeded(gpu.Grid.dense(n), 0, n, n, n, 0, 20260101)
+    _, scale = gpu.quantize_bf16(kg)
+    F = dict(g=g, u=u.astype(np.float32), k=None, bits=gpu.quantize_bf16_scaled(k, scale),
+             factor=float(np.float32(factor * scale)))
+    want, _ = standin_oracle_f32(oracle, F, g, ghost, steps, {"f32c": 1, "f32c16": 2}[prec])
+    got = np.loadtxt(tmp_path / f"u_h3d_slab{part}of4.txt")
+    assert got.size == want.size and np.array_equal(got, want.astype(np.float64))
+
+
 @pytest.mark.parametrize("prec", ["f64", "f32c16"])
 def test_cli_slab_autotune_keeps_bits(tmp_path, prec):
     """--autotune on a slab tunes the interior on the slab pass itself

@@ -161,6 +161,33 @@ def test_quantize_bf16_power_of_two_scale(h3d):
     assert np.array_equal(deq / s, (bits.astype(np.uint32) << 16).view(np.float32))
 
 
+def test_bf16_slab_of_a_field_takes_the_field_scale(h3d):
+    """A slab's kappa quantized with the whole field's scale has the bits of
+    the field's quantization there; quantized with its own maximum's scale
+    it can differ (a slab whose maximum is below half the field's gets a
+    smaller power of two).  The scale is the power of two above max|kappa|."""
+    rng = np.random.default_rng(8)
+    k = rng.uniform(0.1, 3.0, 4000)
+    k[:1000] *= 0.2  # the first "slab" peaks below half the field's maximum
+    bits, s = h3d.quantize_bf16(k)
+    sc = h3d.C.c_double()
+    assert h3d.lib().h3d_bf16_scale(h3d.ptr(k), h3d.C.c_size_t(k.size), h3d.C.byref(sc)) == 0
+    assert sc.value == s == 4.0  # max in [2, 4)
+    slab = k[:1000]
+    assert np.array_equal(h3d.quantize_bf16_scaled(slab, s), bits[:1000])
+    own, s_own = h3d.quantize_bf16(slab)
+    assert s_own == 1.0 and not np.array_equal(own, bits[:1000])
+    np.testing.assert_array_equal(h3d.bf16_to_f64(h3d.quantize_bf16_scaled(slab, s), s),
+                                  h3d.bf16_to_f64(bits[:1000], s))
+    L = h3d.lib()
+    out = np.empty(4, np.uint16)
+    for bad in (3.0, 0.0, -2.0, float("inf")):  # not a positive power of two
+        assert L.h3d_quantize_bf16_scaled(h3d.ptr(k), h3d.ptr(out), h3d.C.c_size_t(4),
+                                          h3d.C.c_double(bad)) == h3d.H3D_EARG
+    assert L.h3d_quantize_bf16_scaled(h3d.ptr(np.array([1e300, 0, 0, 0])), h3d.ptr(out),
+                                      h3d.C.c_size_t(4), h3d.C.c_double(1.0)) == h3d.H3D_EARG
+
+
 def test_write_dump_format(h3d, tmp_path):
     n = 4
     g = h3d.Grid.dense(n)
