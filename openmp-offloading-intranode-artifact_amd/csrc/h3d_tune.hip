@@ -6,7 +6,7 @@
 // only read): launch j reads u_a (j = 0) or the previous launch's output
 // and writes the other scratch field, as the steps of a run do.  Every
 // candidate gets one warm-up launch and the best of three samples of two
-// launches; the four fastest are then timed again over samples of ten
+// launches; the eight fastest are then timed again over samples of ten
 // launches, and the best of those wins (two-launch samples alone favoured
 // a z chunk that runs 8% slower over a 1000-step run).  The figure of
 // merit is time per time step (pass time / kblock).
@@ -257,10 +257,12 @@ extern "C" int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
       }
     }
   }
-  // the four fastest again, over ten launches per sample
+  // the eight fastest again, over ten launches per sample (with four, the
+  // n = 512 reference case with rhs kept a z chunk 1.5% slower than the
+  // best: two-launch samples order shapes within a few percent by chance)
   std::stable_sort(cand.begin(), cand.end(),
                    [](const auto &a, const auto &b) { return a.first < b.first; });
-  for (size_t c = 0; rc == H3D_OK && c < cand.size() && c < 4; ++c) {
+  for (size_t c = 0; rc == H3D_OK && c < cand.size() && c < 8; ++c) {
     float ms;
     const int r = time_best(s, launcher(cand[c].second), 10, &ms);
     if (r != H3D_OK) { rc = r; break; }
