@@ -249,12 +249,12 @@ int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_f32,
  * slab (kappa FP64; mode 0 without rhs, 1 with rhs; ghost 1..4, 1 for
  * h3d_run_slab_f64); kind 1: FP32 slab (kappa FP32, or BF16 bits when
  * kappa_bf16; mode as h3d_pass_f32; ghost 1..8; comp_a, the FP16 residual
- * field, and cface_lo / cface_hi in mode H3D_F32_COMP16).  The rows and
- * waves come from h3d_autotune of the interior launch alone on a grid of
- * exactly its planes; the z chunk from timing whole passes of `ghost`
- * steps (bands, exchange and the interior on a second stream) for that
- * shape's candidate chunks: alone, the interior may prefer another chunk
- * than beside the bands.  Reads u_a, kappa and comp_a only (passes run on
+ * field, and cface_lo / cface_hi in mode H3D_F32_COMP16).  The interior
+ * launch is first tuned alone (h3d_autotune on a grid of exactly its
+ * planes); then whole passes of `ghost` steps (bands, exchange and the
+ * interior on a second stream) are timed for that shape and the
+ * precision's 16-wave fused tiles, each with its candidate z chunks:
+ * alone, the interior may prefer another shape than beside the bands.  Reads u_a, kappa and comp_a only (passes run on
  * scratch copies); face_lo / face_hi (and cface_*) are used as workspace.
  * stream must be a created stream.  Results are cached per (kind, grid,
  * mode, kappa format, ghost); any tune gives the same bits. */

@@ -320,9 +320,10 @@ extern "C" int h3d_run_slab_pass_f32(float *u_a, float *u_b, const float *kappa_
 // marches only the planes between the bands.  Tuned alone on the whole
 // slab, the FP32 3-step interior of the 1026^2 x 262 slab took 3 z chunks
 // (87 planes) and ran 813-817 GLUP/s; two chunks of 125 planes ran 872
-// inside the slab pass.  So the shape (rows, waves) comes from the pass
-// tuned alone on a grid of exactly the interior planes, and its z chunk
-// from timing whole slab passes for that shape's candidate chunks.
+// inside the slab pass, and one chunk of 250 on 48-row tiles 904.  So the
+// pass is first tuned alone on a grid of exactly the interior planes, and
+// then whole slab passes are timed for that shape and the precision's two
+// one-workgroup-per-CU fused tiles, each with its candidate z chunks.
 
 namespace {
 
@@ -375,7 +376,12 @@ extern "C" int h3d_autotune_slab(int kind, const void *u_a, const void *kappa, i
   const size_t ksz = kind ? (kappa_bf16 ? sizeof(uint16_t) : sizeof(float)) : sizeof(double);
   const int nzv = g->nz - 4 * ghost + 2;
   h3d_tune base{};
-  std::vector<int> zcs;
+  std::vector<h3d_tune> cands;
+  auto add = [&](h3d_tune t) {
+    for (const h3d_tune &c : cands)
+      if (c.rows == t.rows && c.waves == t.waves && c.zchunk == t.zchunk) return;
+    cands.push_back(t);
+  };
   if (nzv >= 3) {
     const long long first = (long long)(2 * ghost - 1) * g->sz;
     h3d_grid gv = *g;
@@ -384,35 +390,42 @@ extern "C" int h3d_autotune_slab(int kind, const void *u_a, const void *kappa, i
                                 (const char *)kappa + first * ksz, kappa_bf16, &gv, mode, ghost,
                                 stream, &base);
     if (r0 != H3D_OK) return r0;
-    zcs.push_back(base.zchunk);
-    // 2. the other z chunks for that shape over the interior planes: the
-    //    round model's best three and one chunk of the whole interior
+    add(base);
+    // 2. z chunks over the interior planes -- the round model's best three
+    //    and one chunk of the whole interior -- for that shape and for the
+    //    precision's two one-workgroup-per-CU tiles of fused passes (alone
+    //    the tuner's pick among those flips between boxes and processes,
+    //    while beside the bands they differ by several percent)
     const int planes = nzv - 2;
-    std::vector<int> more;
     if (ghost == 1) {
-      more = {planes, 256, 128, 64};
+      for (int z : {planes, 256, 128, 64})
+        if (z <= planes) add({base.rows, base.waves, z, ghost});
     } else {
-      int occ = 0, z[3], got = 0;
-      long long tiles = 0;
-      const int r = kind ? h3d_pass_f32_geometry(&gv, ghost, base.rows, base.waves, mode,
-                                                 kappa_bf16, &occ, &tiles)
-                         : h3d_pass_f64_geometry(&gv, ghost, base.rows, base.waves, mode, &occ,
-                                                 &tiles);
-      if (r == H3D_OK && h3d_round_zchunks(planes, ghost, tiles, occ, z, 3, &got) == H3D_OK)
-        more.assign(z, z + got);
-      if (planes <= 256) more.push_back(planes);
+      std::vector<std::pair<int, int>> shapes = {{base.rows, base.waves}};
+      if (kind) shapes.insert(shapes.end(), {{3, 16}, {4, 16}});
+      else shapes.insert(shapes.end(), {{2, 16}, {4, 8}});
+      for (const auto &sh : shapes) {
+        int occ = 0, z[3], got = 0;
+        long long tiles = 0;
+        const int r = kind ? h3d_pass_f32_geometry(&gv, ghost, sh.first, sh.second, mode,
+                                                   kappa_bf16, &occ, &tiles)
+                           : h3d_pass_f64_geometry(&gv, ghost, sh.first, sh.second, mode,
+                                                   &occ, &tiles);
+        if (r != H3D_OK) continue;  // shape not offered at this depth
+        if (h3d_round_zchunks(planes, ghost, tiles, occ, z, 3, &got) != H3D_OK) got = 0;
+        for (int i = 0; i < got; ++i) add({sh.first, sh.second, z[i], ghost});
+        if (planes <= 256) add({sh.first, sh.second, planes, ghost});
+      }
     }
-    for (int z : more)
-      if (z >= 1 && z <= planes && std::find(zcs.begin(), zcs.end(), z) == zcs.end())
-        zcs.push_back(z);
   } else {
     const int r0 = h3d_autotune(kind, (void *)u_a, nullptr, kappa, kappa_bf16, g, mode, ghost,
                                 stream, &base);
     if (r0 != H3D_OK) return r0;
+    add(base);
   }
   h3d_tune best = base;
   int rc = H3D_OK;
-  if (zcs.size() > 1) {
+  if (cands.size() > 1) {
     // 3. whole slab passes (bands, exchange, the interior on a second
     //    stream) through scratch copies of u_a (and comp_a): the caller's
     //    fields are only read; the face buffers are workspace
@@ -447,9 +460,8 @@ extern "C" int h3d_autotune_slab(int kind, const void *u_a, const void *kappa, i
                            (uint16_t *)cface_hi, &t, s, s2, fork, join);
     };
     float best_ms = 1e30f;
-    for (size_t k = 0; rc == H3D_OK && k < zcs.size(); ++k) {
-      h3d_tune t = base;
-      t.zchunk = zcs[k];
+    for (size_t k = 0; rc == H3D_OK && k < cands.size(); ++k) {
+      const h3d_tune t = cands[k];
       rc = pass(t, 0);  // warm-up
       // best of three samples of four passes
       for (int r = 0; rc == H3D_OK && r < 3; ++r) {
