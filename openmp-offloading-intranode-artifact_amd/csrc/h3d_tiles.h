@@ -538,8 +538,8 @@ __device__ __forceinline__ void pass_f32_tile(
 // in a register ring of KB planes that levels 2..KB read: a sum formed
 // once has the bits of the same sum formed at every level.  kappa passes
 // through LDS once: plane it is written at iteration it and read at
-// iteration it+1 (a wave's own rows and the edge rows of the waves next
-// to it), so two LDS slots suffice.  With rhs, each lane keeps the rhs of
+// iteration it+1 (the edge rows of the waves next to it, and at depth 2 a
+// wave's own rows), so two LDS slots suffice.  With rhs, each lane keeps the rhs of
 // its cells for the last KB planes in a second register ring (every level
 // adds it; a cell reads only its own).  The march is unrolled by
 // lcm(2, KB) iterations, so every ring slot, level slot and LDS slot is a
@@ -563,6 +563,13 @@ struct Pass64 {
   static constexpr int TY = W * RY;
   static constexpr int SLOT = TY * 64;  // doubles per kappa slot
   static constexpr int U = pass64_unroll(KB);
+  // A wave's own kappa rows of plane it-1 are read from its prefetch
+  // registers instead of back from LDS (only the edge rows then pass
+  // through LDS) at depths 3 and 4: on MI355X at 512^3, 4-step passes on
+  // 8 waves x 4 rows 996 -> 975 us, 3-step passes on 8 x 4 748 -> 741 us,
+  // 3-step passes on 16 x 2 unchanged.  The 2-step pass on 16 x 2 ran
+  // 768 -> 793 us that way, so depth 2 keeps the LDS copy.
+  static constexpr bool KQ_REG = KB >= 3;
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
@@ -622,19 +629,23 @@ struct Pass64 {
   __device__ __forceinline__ void iter(int it) {
 #pragma clang fp contract(off)
     constexpr int P = PH & 1;
-    // 1. kappa of plane it into LDS slot P.  The slot held plane it-2,
-    //    last read at iteration it-1, before its closing barrier.
+    // 1. kappa of plane it into LDS slot P: the edge rows the neighbouring
+    //    waves read, and at KB = 2 every row (see KQ_REG).  The slot held
+    //    plane it-2, last read at iteration it-1, before its closing
+    //    barrier.
 #pragma unroll
-    for (int r = 0; r < RY; ++r) kl[P * SLOT + r * 64] = pk[P][r];
+    for (int r = 0; r < RY; ++r)
+      if (!KQ_REG || r == 0 || r == RY - 1) kl[P * SLOT + r * 64] = pk[P][r];
     // 2. Face sums of plane it-1 (level 1's plane) into ring slot PH-1:
-    //    its kappa rows, and the edge rows of the waves next to this one,
-    //    from LDS slot 1-P (written at iteration it-1); kappa of plane it
-    //    from registers.
+    //    its kappa rows (prefetch slot 1-P, loaded at iteration it-2 and
+    //    refilled by this iteration's prefetch below, or LDS slot 1-P),
+    //    the edge rows of the waves next to this one from LDS slot 1-P
+    //    (written at iteration it-1), kappa of plane it from registers.
     {
       constexpr int c1 = ring(PH - 1);
       double kq[RY];
 #pragma unroll
-      for (int r = 0; r < RY; ++r) kq[r] = kl[(1 - P) * SLOT + r * 64];
+      for (int r = 0; r < RY; ++r) kq[r] = KQ_REG ? pk[1 - P][r] : kl[(1 - P) * SLOT + r * 64];
       const double k_below = kb[(1 - P) * SLOT];
       const double k_above = ka[(1 - P) * SLOT];
 #pragma unroll
@@ -786,6 +797,10 @@ __device__ __forceinline__ void pass_f64_tile(
       st.CX[l][r] = st.CY[l][r] = st.CZ[l][r] = 0.0;
     }
   }
+  // the first iteration forms plane it0-1's face sums from prefetch slot 1
+  // or LDS slot 1 (a warm-up plane: zero in both)
+#pragma unroll
+  for (int r = 0; r < RY; ++r) st.pk[1][r] = 0.0;
   // All LDS starts defined (warm-up iterations read slots not yet written).
   for (int x = threadIdx.x; x < pass64_lds_doubles(KB, RY, W); x += W * 64) lds[x] = 0.0;
   __syncthreads();
