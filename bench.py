@@ -53,6 +53,9 @@ import h3d_py as h3d  # noqa: E402
 
 METRIC = "GLUP/s + % HBM roofline; FP32-compensated vs FP64 at stated rel-L2, 1 MI355X"
 HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E bandwidth
+# MI355X vector issue: 256 CUs x 4 SIMDs at 2.4 GHz, a wave64 VALU
+# instruction (FP64 add/mul included) every 4 cycles per SIMD
+VALU_PEAK_GINST = 256 * 4 * 2.4 / 4
 
 
 def log(*a):
@@ -853,6 +856,29 @@ def main():
               "kernel_ms_per_launch": round(1e3 * per_launch1, 5)}
     log(f"FP64 single steps: {glups1:.1f} GLUP/s ({100 * single['frac_of_stream']:.1f}% of stream)")
 
+    # The configuration's full run length with the headline's tune, once,
+    # outside the timed region: the rate a long run settles at (a short run
+    # pays a partial pass when the step count is not a multiple of the
+    # pass depth).
+    long_steps = 1000
+    d2d(da, d0)
+    d2d(db, d0)
+    cur = (da, db)
+    if a.warmup:
+        cur, _ = run64(da, db, a.warmup, tune)
+    h3d.call("h3d_device_synchronize", 0)
+    t0 = time.perf_counter()
+    _, long_ms = run64(cur[0], cur[1], long_steps, tune)
+    h3d.call("h3d_device_synchronize", 0)
+    long_s = time.perf_counter() - t0
+    long_run = {"steps": long_steps, "value": round(cells * long_steps / long_s / 1e9, 3),
+                "unit": "GLUP/s", "ms_per_step": round(1e3 * long_s / long_steps, 5),
+                "launches": fp64_passes(g, long_steps, tune),
+                "device_ms": round(long_ms, 3),
+                "timed_steps_value_ratio": round(glups / (cells * long_steps / long_s / 1e9), 4)}
+    log(f"FP64 {long_steps} steps (untimed leg): {long_run['value']:.1f} GLUP/s; "
+        f"{a.steps} timed steps reach {100 * long_run['timed_steps_value_ratio']:.1f}% of it")
+
     if kb64 > 1:
         kname = "h3d_pass_f64_kernel"
         key = pass_key(kname, kb64, tune.rows, tune.waves, tune.zchunk, m, ", false")
@@ -899,6 +925,7 @@ def main():
                      "kernel_ms_per_launch": round(1e3 * per_launch_s, 5),
                      "profiled": profiled},
         "fp64_single_step": single,
+        "fp64_1000_steps": long_run,
         "stream": {"copy_gbs": round(copy_gbs, 1), "triad_gbs": round(triad_gbs, 1),
                    "array_bytes": 4 << 30},
         "checksum": {"sum": float(st[0]), "sumsq": float(st[1]), "max": float(st[2]),
@@ -907,6 +934,16 @@ def main():
     if key:
         result["roofline"]["traffic_ratio"] = (traffic_ratio(key, alg_bytes) or {}).get(
             "ratio_to_algorithmic")
+        # the vector-issue ceiling of the same launch: VALU instructions per
+        # launch (SQ_INSTS_VALU of the profiled launch) over this run's
+        # event-timed launch time, against the peak issue rate
+        pe = profile_entry(key, alg_bytes) or {}
+        if pe.get("valu_insts_per_launch") and pe.get("same_shape"):
+            gi = pe["valu_insts_per_launch"] / per_launch_s / 1e9
+            result["roofline"]["valu"] = {
+                "insts_per_launch": pe["valu_insts_per_launch"],
+                "achieved_ginst_s": round(gi, 1), "peak_ginst_s": VALU_PEAK_GINST,
+                "frac": round(gi / VALU_PEAK_GINST, 4), "profile": pe.get("source")}
 
     # ---- FP32 compensated, temporally blocked ----
     if not a.no_fp32:
