@@ -651,11 +651,13 @@ def leg_cfg4(bw, stream):
     return out
 
 
-def leg_cfg5(bw, stream):
+def leg_cfg5(bw, stream, prec32=1):
     """64 independent problems, each axis uniform in [48, 256] interior
     points, lognormal kappa, Gaussian u0, alternating FP64 / FP32-compensated
     members, 500 steps captured in one hipGraph; FP32 members against their
-    FP64 twins."""
+    FP64 twins.  prec32: the FP32 members' compensation, 1 = FP16 residual
+    field across passes, 2 = residual carried in registers within each
+    pass."""
     rng = np.random.default_rng(20260101)
     dims = [tuple(int(x) + 2 for x in rng.integers(48, 257, 3)) for _ in range(64)]
 
@@ -676,7 +678,7 @@ def leg_cfg5(bw, stream):
     for mi, (nx, ny, nz) in enumerate(dims):
         g = h3d.Grid.dense(nx, ny, nz)
         u, k = h3d.gen_seeded(g, 0, nx, ny, nz, 1, 20260101, mi)
-        prec = mi % 2
+        prec = prec32 * (mi % 2)
         if prec:
             u, k = u.astype(np.float32).astype(np.float64), k.astype(np.float32).astype(np.float64)
         h = 1.0 / (nx - 1)
@@ -719,16 +721,17 @@ def leg_cfg5(bw, stream):
     # cell and pass of kblock steps
     k64 = et.kblock
     b64 = sum(24.0 / k64 * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
-    b32 = sum(16.0 / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
+    b32c = 16.0 if prec32 == 1 else 12.0
+    b32 = sum(b32c / et.kblock * (x - 2) * (y - 2) * (z - 2) for x, y, z in dims[1::2])
     gbs = (b64 + b32) * steps / sec / 1e9
     # one launch per group of kblock steps: the pass over every member
     # (profile key by kernel name: the grid is the device's resident
     # workgroups)
     c64 = sum((x - 2) * (y - 2) * (z - 2) for x, y, z in dims[0::2])
     c32 = cells - c64
-    alg = 24.0 * c64 + 16.0 * c32
+    alg = 24.0 * c64 + b32c * c32
     key = f"ensemble_step<{et.kblock}>"
-    note_launch("ensemble", key, alg)
+    note_launch("ensemble" if prec32 == 1 else "ensemble_fp32_carry", key, alg)
     traffic = {"launch": key, "algorithmic_bytes_per_launch": alg,
                "traffic": traffic_ratio(key, alg)}
     return {"value": round(glups, 3), "unit": "GLUP/s (aggregate)", "members": 64,
@@ -737,8 +740,18 @@ def leg_cfg5(bw, stream):
             "ms_per_step": round(1e3 * sec / steps, 5),
             "work_items": nwork.value, "workgroups": nblk.value,
             "achieved_gbs": round(gbs, 1), "frac_of_stream": round(gbs / bw, 4),
+            "fp32_members": "FP16 residual field" if prec32 == 1 else "register carry",
             "worst_fp32_member_rel_l2_vs_fp64": worst,
             "fp32_members_vs_fp64": per_member}
+
+
+def leg_cfg5_both(bw, stream):
+    """The ensemble configuration with FP32 members whose residual is
+    carried in registers through each 3-step pass, and beside it the same
+    members with the FP16 residual field carried across passes."""
+    r = leg_cfg5(bw, stream, 2)
+    r["fp32_fp16_residual"] = leg_cfg5(bw, stream, 1)
+    return r
 
 
 def main():
@@ -1069,7 +1082,7 @@ def main():
                                  leg_reference_workload),
                                 ("slab", "slab_1024x1024x256_fp64", leg_cfg3),
                                 ("cfg4", "fp32c_1024cubed_2000steps", leg_cfg4),
-                                ("ensemble", "ensemble_64_members_500steps", leg_cfg5)):
+                                ("ensemble", "ensemble_64_members_500steps", leg_cfg5_both)):
             if only and short not in only:
                 continue
             log(f"{name} ...")

@@ -368,20 +368,23 @@ int h3d_autotune(int kind, void *u_a, void *u_b, const void *kappa,
 
 typedef struct h3d_member {
   void *u_a, *u_b;   /* device: ping-pong fields, FP64 or FP32, ring seeded */
-  const void *kappa; /* device: FP64 (precision 0) or FP32 (precision 1)    */
+  const void *kappa; /* device: FP64 (precision 0) or FP32 (precision 1, 2) */
   h3d_grid g;        /* member grid                                         */
   double factor;     /* dt/h/h/2 of this member (FP32 members use it
                         rounded to float)                                   */
   int precision;     /* 0 = FP64 step (as h3d_step_f64); 1 = FP32 storage,
                         compensated, with an FP16 residual field kept by the
                         ensemble across passes (as h3d_pass_f32 with mode
-                        H3D_F32_COMP16 and the ensemble's kblock)           */
+                        H3D_F32_COMP16 and the ensemble's kblock); 2 = FP32
+                        storage, compensated, the residual carried in
+                        registers through each pass's fused steps (as
+                        h3d_pass_f32 with mode H3D_F32_CARRY)               */
 } h3d_member;
 
 typedef struct h3d_ensemble h3d_ensemble;
 
 /* Builds the tile work lists on `device_id` and allocates the FP32
- * members' residual fields (zeroed).  Tiles are fixed in x and y (1024-
+ * members' residual fields (zeroed; precision 1 only).  Tiles are fixed in x and y (1024-
  * thread workgroups of 32 rows for FP64 members and 48 rows for FP32
  * members; one persistent kernel serves both).  tune may be NULL;
  * tune->zchunk, if non-zero, is the z-march

@@ -14,7 +14,9 @@
 //    and the FP64 pass tile for passes (bit-identical to h3d_pass_f64);
 //  - FP32 members: 1024-thread workgroups of 48 rows (16 waves x 3 rows),
 //    the FP32 pass tile with the FP16 residual field (mode
-//    H3D_F32_COMP16), so the compensation carries across passes; a pass
+//    H3D_F32_COMP16, precision 1), so the compensation carries across
+//    passes, or with the residual carried in registers through each
+//    pass's fused steps (mode H3D_F32_CARRY, precision 2); a pass
 //    fuses `kblock` steps, bit-identical to h3d_pass_f32 with the same
 //    nsteps (any shape: the tile changes the schedule, not the
 //    arithmetic).  Against 512-thread workgroups of 32 rows (8 waves x 4
@@ -127,7 +129,7 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
                                             m.nx, m.ny, m.sy, m.sz, m.factor, 1,
                                             m.nz - 2, zchunk, 1, d.y, d.z, d.w);
         }
-      } else {
+      } else if (m.precision == 1) {
         const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
         float *out = static_cast<float *>(parity ? m.ua : m.ub);
         const unsigned short *cin = parity ? m.cb : m.ca;
@@ -135,6 +137,14 @@ ensemble_step(const EMember *__restrict__ mem, const int4 *__restrict__ work,
         const int zb = 1 + d.w * zchunk;
         pass_f32_tile<KB, ENS32_RY, ENS_WAVES, 2, false>(
             lds, out, in, static_cast<const float *>(m.kap), nullptr, cout, cin,
+            m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
+            min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
+      } else {  // precision 2: the residual carried in registers
+        const float *in = static_cast<const float *>(parity ? m.ub : m.ua);
+        float *out = static_cast<float *>(parity ? m.ua : m.ub);
+        const int zb = 1 + d.w * zchunk;
+        pass_f32_tile<KB, ENS32_RY, ENS_WAVES, 1, false>(
+            lds, out, in, static_cast<const float *>(m.kap), nullptr, nullptr, nullptr,
             m.nx, m.ny, m.nz, m.sy, m.sz, m.factor32, zb,
             min(m.nz - 2, zb + zchunk - 1), d.y, d.z);
       }
@@ -252,7 +262,7 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
         // a tile's planes, weighted by the measured time of one plane
         // iteration: 2.4 us for the 48-row FP32 pass tile against 1.7 us
         // for the 32-row FP64 tile (depth 3, 64-member ensemble, MI355X)
-        const long long cost = (long long)planes * (t.precision ? 7 : 5);
+        const long long cost = (long long)planes * (t.precision == 1 ? 7 : 5);
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
       }
@@ -371,7 +381,7 @@ extern "C" int h3d_ensemble_create(h3d_ensemble **out,
     const h3d_member &x = members[m];
     if (!x.u_a || !x.u_b || !x.kappa || x.u_a == x.u_b || !h3d_grid_ok(&x.g))
       return H3D_EARG;
-    if (x.precision != 0 && x.precision != 1) return H3D_EARG;
+    if (x.precision < 0 || x.precision > 2) return H3D_EARG;
   }
   h3d_ensemble *e = new h3d_ensemble();
   e->device = device_id;

@@ -1,9 +1,11 @@
 """Ensembles on the GPU: every member of a mixed FP64 / FP32-compensated
 ensemble advanced by the persistent work-list kernel equals the same
-member advanced alone (FP64: h3d_run_f64; FP32: the CPU oracle's FP16
-residual mode, one step per pass), bit for bit, eagerly and from a
-hipGraph; and the 64-member configuration keeps every FP32 member within
-rel-L2 1e-5 of its FP64 twin after 500 steps."""
+member advanced alone (FP64: h3d_run_f64; FP32: the CPU oracle replaying
+the same passes, with the FP16 residual field carried across them
+(precision 1) or the residual carried in registers within each pass
+(precision 2)), bit for bit, eagerly and from a hipGraph; and the
+64-member configuration keeps every FP32 member within rel-L2 1e-5 of its
+FP64 twin after 500 steps."""
 import ctypes as C
 
 import numpy as np
@@ -57,7 +59,25 @@ def run(h3d, e, steps, graph, stream):
 
 
 SPECS = [((5, 5, 5), 0), ((70, 9, 40), 1), ((66, 66, 66), 0), ((48, 131, 57), 1),
-         ((130, 50, 3 + 2), 0), ((3, 80, 70), 1), ((100, 33, 150), 1)]
+         ((130, 50, 3 + 2), 0), ((3, 80, 70), 1), ((100, 33, 150), 1), ((40, 70, 33), 2),
+         ((81, 5, 66), 2)]
+
+
+def f32_mode(precision):
+    """The FP32 pass mode of an ensemble member's precision."""
+    return 2 if precision == 1 else 1
+
+
+def oracle_f32_passes(oracle, sp, depths):
+    """The oracle replaying an FP32 member's passes: the FP16 residual
+    carried across passes (precision 1), or reset at each pass (2)."""
+    g = sp["g"]
+    f32 = float(np.float32(sp["factor"]))
+    u, comp = sp["u"], None
+    for d in depths:
+        u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d,
+                                 f32_mode(sp["precision"]), comp0=comp)
+    return u
 
 
 @pytest.mark.parametrize("graph", [0, 1])
@@ -79,9 +99,7 @@ def test_members_equal_standalone_runs(gpu, oracle, graph):
                                   g.nz, steps, factor=sp["factor"])
             assert np.array_equal(got, want), g.nx
         else:
-            f32 = float(np.float32(sp["factor"]))
-            want, _ = oracle.run_f32(sp["u"], sp["k"], None, g.nx, g.ny, g.nz, f32, steps,
-                                     1, 2)
+            want = oracle_f32_passes(oracle, sp, [1] * steps)
             assert got.tobytes() == want.tobytes(), g.nx
     gpu.call("h3d_ensemble_destroy", e)
 
@@ -119,11 +137,7 @@ def test_zchunk_and_autotune_keep_results(gpu, oracle, zchunk):
                                   g.nz, steps, factor=sp["factor"])
             assert np.array_equal(got, want), g.nx
         else:
-            f32 = float(np.float32(sp["factor"]))
-            u, comp = sp["u"], None
-            for d in depths:
-                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
-                                         comp0=comp)
+            u = oracle_f32_passes(oracle, sp, depths)
             assert got.tobytes() == u.tobytes(), (g.nx, out.kblock)
     gpu.call("h3d_ensemble_destroy", e)
 
@@ -139,7 +153,7 @@ def test_fp32_pass_depth_members_equal_standalone(gpu, oracle, kb, graph):
     check_members_over_runs(gpu, oracle, specs, kb, graph)
 
 
-@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_single_precision_ensembles_equal_standalone(gpu, oracle, prec):
     """One persistent kernel serves FP64 and FP32 tiles; an ensemble whose
     members all have one precision takes only that path, and each member
@@ -178,11 +192,7 @@ def check_members_over_runs(gpu, oracle, specs, kb, graph):
             assert np.array_equal(got, want), g.nx
         else:
             assert where(gpu, e, m) == len(depths) % 2
-            f32 = float(np.float32(sp["factor"]))
-            u, comp = sp["u"], None
-            for d in depths:
-                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
-                                         comp0=comp)
+            u = oracle_f32_passes(oracle, sp, depths)
             assert got.tobytes() == u.tobytes(), (g.nx, kb)
     gpu.call("h3d_ensemble_destroy", e)
 
@@ -216,13 +226,14 @@ def cfg5_dims(seed=20260101, count=64):
     return [tuple(int(x) + 2 for x in rng.integers(48, 257, 3)) for _ in range(count)]
 
 
-def test_cfg5_64_members_500_steps_accuracy(gpu):
+@pytest.mark.parametrize("prec32", [1, 2])
+def test_cfg5_64_members_500_steps_accuracy(gpu, prec32):
     """64 members, each axis uniform in [48, 256] interior points, lognormal
-    kappa, Gaussian u0, alternating FP64 / FP32-compensated, 500 steps from
-    one hipGraph; each FP32 member against the same member run in FP64 on
-    its FP32-rounded inputs."""
+    kappa, Gaussian u0, alternating FP64 / FP32-compensated (FP16 residual
+    field, or register carry), 500 steps from one hipGraph; each FP32 member
+    against the same member run in FP64 on its FP32-rounded inputs."""
     dims = cfg5_dims()
-    specs = [make_member(gpu, d, m, 20260101, m % 2) for m, d in enumerate(dims)]
+    specs = [make_member(gpu, d, m, 20260101, prec32 * (m % 2)) for m, d in enumerate(dims)]
     e, keep = build(gpu, specs)
     s = gpu.Stream()
     ms = run(gpu, e, 500, 1, s)
@@ -284,11 +295,7 @@ def test_cfg5_members_bitwise_first_steps(gpu, oracle):
                                   g.nz, steps, factor=sp["factor"], nthreads=16)
             assert np.array_equal(got, want), (m, dims[m])
         else:
-            f32 = float(np.float32(sp["factor"]))
-            u, comp = sp["u"], None
-            for d in depths:
-                u, comp = oracle.run_f32(u, sp["k"], None, g.nx, g.ny, g.nz, f32, d, d, 2,
-                                         comp0=comp)
+            u = oracle_f32_passes(oracle, sp, depths)
             assert got.tobytes() == u.tobytes(), (m, dims[m])
             checked32 += 1
     assert checked32 == 8
