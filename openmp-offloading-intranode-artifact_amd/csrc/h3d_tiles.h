@@ -578,8 +578,11 @@ struct Pass64 {
   // scheduling barrier after each level keeps the register use near the
   // branchy form's.  Measured at 512^3: 4-step passes on 8 waves x 4 rows
   // 1046-1056 -> 996-1001 us per launch, 3-step passes 0.8-1% faster; the
-  // 2-step passes ran 2% slower, so depth 2 keeps the branches.
-  static constexpr bool SELECT_OPAQUE = KB >= 3;
+  // 2-step passes ran 2% slower, so depth 2 keeps the branches.  With rhs
+  // (a ring of rhs values in registers) the form spills: 3-step passes on
+  // 16 waves x 2 rows 1178 -> 2221 us, 4-step on 8 x 4 3650 -> 6129 us
+  // (3-step on 8 x 4: 1015 -> 1007), so passes with rhs keep the branches.
+  static constexpr bool SELECT_OPAQUE = KB >= 3 && !RHS;
   // ---- per-lane state ----
   double A[KB][2][RY];    // u of level l: slot P = plane it-l-1, 1-P = it-l
   double FZ[KB + 1][RY];  // z flux above the last plane of level l
