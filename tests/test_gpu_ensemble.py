@@ -263,14 +263,16 @@ def test_cfg5_64_members_500_steps_accuracy(gpu, prec32):
     gpu.call("h3d_ensemble_destroy", e2)
 
 
-def test_cfg5_members_bitwise_first_steps(gpu, oracle):
+@pytest.mark.parametrize("prec32", [1, 2])
+def test_cfg5_members_bitwise_first_steps(gpu, oracle, prec32):
     """The 64-member configuration at full size with every member fused 3
     steps per pass (the depth the benchmark's tuner picks): after 7 steps
     (two 3-step passes and a single step) every FP64 member equals the
     oracle's run bit for bit, and a sample of FP32 members equals the
-    oracle replaying the same passes with the FP16 residual carried."""
+    oracle replaying the same passes, with the FP16 residual carried across
+    them (precision 1) or reset at each pass (precision 2)."""
     dims = cfg5_dims()
-    specs = [make_member(gpu, d, m, 20260101, m % 2) for m, d in enumerate(dims)]
+    specs = [make_member(gpu, d, m, 20260101, prec32 * (m % 2)) for m, d in enumerate(dims)]
     mems, keep = [], []
     for sp in specs:
         da, db, dk = (gpu.DeviceArray.from_host(x) for x in (sp["u"], sp["u"], sp["k"]))
@@ -287,7 +289,7 @@ def test_cfg5_members_bitwise_first_steps(gpu, oracle):
     checked32 = 0
     for m, (sp, (da, db, dk)) in enumerate(zip(specs, keep)):
         g = sp["g"]
-        if sp["precision"] == 1 and m % 8 != 1:
+        if sp["precision"] != 0 and m % 8 != 1:
             continue  # FP32 sample: members 1, 9, 17, ... (the oracle's FP32 path is serial)
         got = (db if where(gpu, e, m) else da).download()
         if sp["precision"] == 0:
