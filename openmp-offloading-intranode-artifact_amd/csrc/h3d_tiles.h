@@ -539,9 +539,9 @@ __device__ __forceinline__ void pass_f32_tile(
 // once has the bits of the same sum formed at every level.  kappa passes
 // through LDS once: plane it is written at iteration it and read at
 // iteration it+1 (the edge rows of the waves next to it, and at depth 2 a
-// wave's own rows), so two LDS slots suffice.  With rhs, each lane keeps the rhs of
-// its cells for the last KB planes in a second register ring (every level
-// adds it; a cell reads only its own).  The march is unrolled by
+// wave's own rows), so two LDS slots suffice.  With rhs, each lane keeps
+// the rhs of its cells for the last KB planes in a second register ring
+// (every level adds it; a cell reads only its own).  The march is unrolled by
 // lcm(2, KB) iterations, so every ring slot, level slot and LDS slot is a
 // compile-time index.
 
