@@ -260,8 +260,10 @@ static int build_work(h3d_ensemble *e, int zchunk, int kb) {
       for (int tz = 0; tz < chunks; ++tz) {
         const int planes = std::min(zchunk, t.planes - tz * zchunk);
         // a tile's planes, weighted by the measured time of one plane
-        // iteration: 2.4 us for the 48-row FP32 pass tile against 1.7 us
-        // for the 32-row FP64 tile (depth 3, 64-member ensemble, MI355X)
+        // iteration: 2.4 us for the 48-row FP32 pass tile with the FP16
+        // residual field against 1.7 us for the 32-row FP64 tile (depth 3,
+        // 64-member ensemble, MI355X); the FP32 carry tile, without the
+        // residual field, is weighted as the FP64 tile
         const long long cost = (long long)planes * (t.precision == 1 ? 7 : 5);
         for (int ty = 0; ty < ty_n; ++ty)
           for (int tx = 0; tx < tx_n; ++tx) jobs.push_back({cost, {m, tx, ty, tz}});
