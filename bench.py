@@ -936,7 +936,12 @@ def main():
                      "algorithmic_bytes_per_launch": alg_bytes,
                      "algorithmic_bytes_per_lup": round(24.0 / kb64, 3),
                      "kernel_ms_per_launch": round(1e3 * per_launch_s, 5),
-                     "profiled": profiled},
+                     "profiled": profiled,
+                     # the bound of any kernel that makes one HBM pass per
+                     # step (24 B per update at the peak), which fused
+                     # passes are not held to
+                     "single_step_bound_glups": round(HBM_PEAK_GBS / 24.0, 1),
+                     "value_over_single_step_bound": round(glups / (HBM_PEAK_GBS / 24.0), 3)},
         "fp64_single_step": single,
         "fp64_1000_steps": long_run,
         "stream": {"copy_gbs": round(copy_gbs, 1), "triad_gbs": round(triad_gbs, 1),
