@@ -1069,7 +1069,7 @@ def main():
         legs = {k: v for k, v in result.items()
                 if k.startswith("fp32c") and isinstance(v, dict) and v.get("targets")}
         met = [k for k, v in legs.items() if all(v["targets"].values())]
-        result["fp32c_north_star"] = {
+        result["fp32c_target"] = {
             "target": ">=0.8 of measured stream, >=1.8x the single-step FP64 kernel, "
                       "rel-L2 <= 1e-5 vs FP64 after 1000 steps, temporally blocked (k>=2)",
             "legs_meeting_all": met, "met": bool(met),
