@@ -77,7 +77,7 @@ def parse():
                     help="steps of each configuration leg (0: the configuration's own; "
                          "profiling runs use a few, since counters are per launch)")
     ap.add_argument("--only", default="",
-                    help="comma list of configuration legs to run: reference,slab,cfg4,ensemble")
+                    help="comma list of configuration legs to run: reference,slab,cube1024,ensemble")
     ap.add_argument("--no-fp32", action="store_true", help="skip the FP32 leg")
     ap.add_argument("--dense", action="store_true",
                     help="reference's dense layout instead of 128-byte aligned rows")
@@ -374,7 +374,7 @@ def leg_reference_workload(bw, stream):
             "ratio_to_reference_mi250x_hip": round(glups / mi250x_hip, 2)}
 
 
-def leg_cfg3(bw, stream):
+def leg_slab(bw, stream):
     """One GPU's slab of a 4-way z split of 1024^3: 1024 x 1024 x 256 owned
     planes + 2 ghosts (1026 x 1026 x 258 points), lognormal kappa, 500 FP64
     steps, boundary planes and in-HBM face pack/unpack overlapped with the
@@ -522,7 +522,7 @@ def slab_deep_f32(n, owned, gk_owned, K, factor, steps, bw, stream):
             "launch": key, "traffic": traffic_ratio(key, alg)}
 
 
-def leg_cfg4(bw, stream):
+def leg_cube1024(bw, stream):
     """1024^3 interior, FP32 storage, 2000 steps, each run against the FP64
     kernel run on the same (rounded / dequantised) inputs:
       - compensated and temporally blocked at the autotuner's depth, kappa
@@ -603,8 +603,8 @@ def leg_cfg4(bw, stream):
             key = march_key("h3d_march_f32", t.rows, t.waves, t.zchunk, n, n, m, extra)
         r["launch"] = key
         r["traffic"] = traffic_ratio(key, bpl * m ** 3)
-        note_launch(f"cfg4_{label}", key, bpl * m ** 3)
-        log(f"cfg4 {label}: {r}")
+        note_launch(f"cube1024_{label}", key, bpl * m ** 3)
+        log(f"1024^3 {label}: {r}")
         return r
 
     fk32 = h3d.AlignedField(g, np.float32)
@@ -617,7 +617,7 @@ def leg_cfg4(bw, stream):
                        ", false")
         out["f32_kappa_fp64_reference_launch"] = k64
         out["f32_kappa_fp64_reference_traffic"] = traffic_ratio(k64, 24.0 * m ** 3)
-        note_launch("cfg4_fp64_reference", k64, 24.0 * m ** 3)
+        note_launch("cube1024_fp64_reference", k64, 24.0 * m ** 3)
     f32f = h3d.C.c_float(np.float32(factor))
     out["f32_kappa_carry"] = leg("f32_kappa_carry", fk32, fk32.ptr, None, f32f, 1, 0, ref, 4.0)
     out["f32_kappa_fp16_residual"] = leg("f32_kappa_fp16_residual", fk32, fk32.ptr, None,
@@ -651,7 +651,7 @@ def leg_cfg4(bw, stream):
     return out
 
 
-def leg_cfg5(bw, stream, prec32=1):
+def leg_ensemble(bw, stream, prec32=1):
     """64 independent problems, each axis uniform in [48, 256] interior
     points, lognormal kappa, Gaussian u0, alternating FP64 / FP32-compensated
     members, 500 steps captured in one hipGraph; FP32 members against their
@@ -745,12 +745,12 @@ def leg_cfg5(bw, stream, prec32=1):
             "fp32_members_vs_fp64": per_member}
 
 
-def leg_cfg5_both(bw, stream):
+def leg_ensemble_both(bw, stream):
     """The ensemble configuration with FP32 members whose residual is
     carried in registers through each 3-step pass, and beside it the same
     members with the FP16 residual field carried across passes."""
-    r = leg_cfg5(bw, stream, 2)
-    r["fp32_fp16_residual"] = leg_cfg5(bw, stream, 1)
+    r = leg_ensemble(bw, stream, 2)
+    r["fp32_fp16_residual"] = leg_ensemble(bw, stream, 1)
     return r
 
 
@@ -1085,9 +1085,9 @@ def main():
         only = set(x for x in a.only.split(",") if x)
         for short, name, fn in (("reference", "reference_workload_n512_501steps",
                                  leg_reference_workload),
-                                ("slab", "slab_1024x1024x256_fp64", leg_cfg3),
-                                ("cfg4", "fp32c_1024cubed_2000steps", leg_cfg4),
-                                ("ensemble", "ensemble_64_members_500steps", leg_cfg5_both)):
+                                ("slab", "slab_1024x1024x256_fp64", leg_slab),
+                                ("cube1024", "fp32c_1024cubed_2000steps", leg_cube1024),
+                                ("ensemble", "ensemble_64_members_500steps", leg_ensemble_both)):
             if only and short not in only:
                 continue
             log(f"{name} ...")

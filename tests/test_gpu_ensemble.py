@@ -221,18 +221,18 @@ def test_ensemble_rejects_bad_members(gpu):
     assert gpu.lib().h3d_ensemble_create(C.byref(e), bad2, 1, None, 0) == -1
 
 
-def cfg5_dims(seed=20260101, count=64):
+def ensemble64_dims(seed=20260101, count=64):
     rng = np.random.default_rng(seed)
     return [tuple(int(x) + 2 for x in rng.integers(48, 257, 3)) for _ in range(count)]
 
 
 @pytest.mark.parametrize("prec32", [1, 2])
-def test_cfg5_64_members_500_steps_accuracy(gpu, prec32):
+def test_ensemble64_500_steps_accuracy(gpu, prec32):
     """64 members, each axis uniform in [48, 256] interior points, lognormal
     kappa, Gaussian u0, alternating FP64 / FP32-compensated (FP16 residual
     field, or register carry), 500 steps from one hipGraph; each FP32 member
     against the same member run in FP64 on its FP32-rounded inputs."""
-    dims = cfg5_dims()
+    dims = ensemble64_dims()
     specs = [make_member(gpu, d, m, 20260101, prec32 * (m % 2)) for m, d in enumerate(dims)]
     e, keep = build(gpu, specs)
     s = gpu.Stream()
@@ -264,14 +264,14 @@ def test_cfg5_64_members_500_steps_accuracy(gpu, prec32):
 
 
 @pytest.mark.parametrize("prec32", [1, 2])
-def test_cfg5_members_bitwise_first_steps(gpu, oracle, prec32):
+def test_ensemble64_members_bitwise_first_steps(gpu, oracle, prec32):
     """The 64-member configuration at full size with every member fused 3
     steps per pass (the depth the benchmark's tuner picks): after 7 steps
     (two 3-step passes and a single step) every FP64 member equals the
     oracle's run bit for bit, and a sample of FP32 members equals the
     oracle replaying the same passes, with the FP16 residual carried across
     them (precision 1) or reset at each pass (precision 2)."""
-    dims = cfg5_dims()
+    dims = ensemble64_dims()
     specs = [make_member(gpu, d, m, 20260101, prec32 * (m % 2)) for m, d in enumerate(dims)]
     mems, keep = [], []
     for sp in specs:

@@ -163,7 +163,7 @@ def test_graph_replay_equals_eager(gpu):
     assert np.array_equal(res[1], res[3])
 
 
-def test_cfg1_128_cubed_100_steps(gpu, oracle):
+def test_128_cubed_100_steps(gpu, oracle):
     """128^3 interior, seeded sine kappa, dt = 0.9 * bound, 100 steps: the
     GPU must agree with the CPU oracle to <= 1e-12 max relative error; with
     contraction off and the same association order it is bit-exact."""
