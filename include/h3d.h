@@ -120,9 +120,12 @@ int h3d_pass_f64(double *u_out, const double *u_in, const double *rhs,
  * has at least r passes and h3d_autotune has timed depths K and K+1 on
  * this grid (it does so when it picks K), the last r passes are K+1 steps
  * deep instead (the tuner's K+1 shape) if by those times that saves more
- * than half a K-step pass, one pass fewer.  The bits are those of `steps` single steps either way.  graph != 0 captures the run's
- * passes (up to 1024 of them; longer runs replay a graph of 1024 K-step
- * passes) in a hipGraph and launches it.  *ms receives
+ * than half a K-step pass, one pass fewer.  A 1-step remainder runs as a
+ * single step on the single-step shape h3d_autotune found on this grid
+ * (with fixed_kblock 1), else on that kernel's default shape.  The bits
+ * are those of `steps` single steps either way.  graph != 0 captures the
+ * run's passes (up to 1024 of them; longer runs replay a graph of 1024
+ * K-step passes) in a hipGraph and launches it.  *ms receives
  * the device time of the loop measured with HIP events on `stream`;
  * *final_in_b is 1 if the result is in u_b (an odd number of passes). */
 int h3d_run_f64(double *u_a, double *u_b, const double *rhs,
